@@ -1,0 +1,60 @@
+"""Named workloads of the explicit RK4 step, as ns_config structures.
+
+The reference's coordinates start at 0 (src/SmallVectors.h:60-62), so a domain [-a, a]^3 maps to
+[0, 2a]^3 with the body centre shifted by +a. The reference's Reynolds number is rho_0 c_0 L / mu_0
+(ConfigFile:109-110), so a body Reynolds number Re_D at Mach M and diameter D is Re = Re_D / (M D).
+"Slip side walls" are symmetry planes on y and z; x is inlet (min) / outlet (max). CFL 0.5 is
+convStabilityLimit; viscStabilityLimit stays at the ConfigFile's -0.68; the periodic density filter is off.
+"""
+from __future__ import annotations
+
+import nsfsi
+
+SIDE_WALLS = ["inlet", "outlet", "symmetry", "symmetry", "symmetry", "symmetry"]
+
+
+def _base(N, L, Re_D, M, D, steps, **kw):
+    return nsfsi.make_config(
+        NI=N, NJ=N, NK=N, L_x=L, L_y=L, L_z=L,
+        Re=Re_D / (M * D), M_0=M, gamma=1.4, Pr=0.72,
+        conv_stability_limit=0.5, visc_stability_limit=-0.68,
+        stop_criterion="timesteps", stop_time_level=steps,
+        filter_intervals=[0], filter_change_times=[],
+        conv_history="all", bc=SIDE_WALLS, **kw)
+
+
+def sphere48(N=48, steps=200, precision="fp64"):
+    """48^3 on [-2,2]^3, sphere r=0.5 at the origin, Re_D=100, M=0.3, Gaussian sigma 1e-3 (seed 1)."""
+    return _base(N, 4.0, 100.0, 0.3, 1.0, steps, body_type="sphere", body_centre=(2.0, 2.0, 2.0), body_radius=0.5,
+                 ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=1, precision=precision)
+
+
+def cylinder256(N=256, steps=1000, precision="fp64"):
+    """256^3 on [-4,4]^3, z-cylinder r=0.5 at the origin spanning z, Re_D=200, M=0.2, U[-1e-3,1e-3] (seed 2)."""
+    return _base(N, 8.0, 200.0, 0.2, 1.0, steps, body_type="cylinder", body_axis="z", body_centre=(4.0, 4.0, 0.0),
+                 body_radius=0.5, ic_type="freestream_uniform", ic_amplitude=1e-3, ic_seed=2, precision=precision)
+
+
+def sphere512(N=512, steps=1000, precision="fp32"):
+    """512^3 on [-4,4]^3, sphere r=0.6 at (-1,0,0), Re_D=300, M=0.3, Gaussian sigma 1e-3 (seed 3)."""
+    return _base(N, 8.0, 300.0, 0.3, 1.2, steps, body_type="sphere", body_centre=(3.0, 4.0, 4.0), body_radius=0.6,
+                 ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=3, precision=precision)
+
+
+def sphere96(N=96, steps=400, precision="fp64"):
+    """96^3 on [-2,2]^3, sphere r=0.5 at the origin, Re_D=100, M=0.3, Gaussian sigma 1e-3 (seed 5)."""
+    return _base(N, 4.0, 100.0, 0.3, 1.0, steps, body_type="sphere", body_centre=(2.0, 2.0, 2.0), body_radius=0.5,
+                 ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=5, precision=precision)
+
+
+def reference_default():
+    """The reference's shipped ConfigFile: 161x83x3 quasi-2-D cylinder channel, periodic z (ConfigFile:22-125)."""
+    return nsfsi.make_config()
+
+
+WORKLOADS = {
+    "sphere48": sphere48,
+    "cylinder256": cylinder256,
+    "sphere512": sphere512,
+    "sphere96": sphere96,
+}

@@ -1,0 +1,102 @@
+/*
+ * case.h — one simulation case on one GPU (host object behind an ns_handle).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "ns_internal.h"
+#include "stage.h"
+
+namespace nsfsi {
+
+struct ConfigError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct GeometryError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct HipError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct StateError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct Unsupported : std::runtime_error { using std::runtime_error::runtime_error; };
+
+void hipCheck(hipError_t e, const char* what);
+
+class Case {
+public:
+    explicit Case(const ns_config& c);
+    ~Case();
+    Case(const Case&) = delete;
+    Case& operator=(const Case&) = delete;
+
+    void init();
+    void setInitialField(const double* U5);
+    void step(int nsteps);
+    void getField(int which, double* out);
+    void nodeTypes(uint8_t* out);
+    void ghosts(int32_t* ijk, double* bi, double* ip, int32_t* box);
+    void filterNodes(int32_t* ijk);
+    int64_t interiorCount();
+    void totals(double out[5]);
+    void readScalars();
+    bool filterDisabled() const;
+    // Runs nsteps steps with HIP events around each kernel group; ms[g] receives the mean duration of
+    // one launch of group g (0 stage kernel, 1 primitive pass, 2 boundary kernels of one stage,
+    // 3 end-of-step reductions) and launches[g] the launches per step.
+    void profile(int nsteps, double ms[4], int launches[4]);
+
+    ns_config cfg;
+    bool fp32 = false;
+    size_t eb = 8;
+    bool initialized = false;
+    hipStream_t stream = nullptr;
+    Layout L{};
+    Params P{};
+    Window W{};
+    BodyDesc B{};
+    EdgeFace faces[6]{};  // application order (reverse of the claim order)
+
+    // classification results
+    uint8_t* cls = nullptr;
+    int64_t nGhost = 0, nFilter = 0, nInactive = 0;
+    int64_t* ghostList = nullptr;
+    int64_t* filterList = nullptr;
+    int64_t* inactiveList = nullptr;
+    double* inactiveIC = nullptr;
+    GhostTables G{};
+
+    // fields (element type float or double)
+    void *U = nullptr, *A = nullptr, *S1 = nullptr, *S2 = nullptr, *prim = nullptr;
+    void *rhoPre = nullptr, *rhoEPre = nullptr, *ftmp = nullptr;
+
+    // step scalars and reductions
+    StepScalars* scal = nullptr;
+    double* partials = nullptr;
+    int nPartials = 0;
+    double* hist = nullptr;
+    int histCap = 1024;
+    std::vector<double> normsHost[5];
+    double t = 0, dt = 0;
+    uint64_t level = 0;
+    bool nonFinite = false;
+
+private:
+    void* alloc(size_t bytes);
+    void setupBoundaries();
+    void categorizeNodes();
+    template <class T> void allocFields();
+    template <class T> void hostInitialField(std::vector<T>& h) const;
+    template <class T> void uploadState(const std::vector<T>& h);
+    template <class T> void computeDt(int mode);
+    template <class T> void applyBCs(T* S, double frac);
+    template <class T> void stepOnce();
+    std::vector<void*> allocations;
+    struct EventPair { hipEvent_t a, b; int group; };
+    std::vector<EventPair> events;
+    bool profiling = false;
+    void mark(int group, bool begin);
+};
+
+}  // namespace nsfsi

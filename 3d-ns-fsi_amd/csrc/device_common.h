@@ -1,0 +1,74 @@
+/*
+ * device_common.h — device-side equation of state and small helpers, templated on the field type.
+ *
+ * Nondimensional perturbation variables as in the reference (src/FlowVariableGroupStructs.h:152-185):
+ * rho' = rho/rho_0 - 1, velocities in units of c_0, p' and T' perturbations, rho*E' without the
+ * constant 1/(gamma(gamma-1)).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ns_internal.h"
+
+namespace nsfsi {
+
+template <class T>
+struct Eos {
+    T gamma, gm1, ScPlusOne, Re, pf;  // pf = 1 / ((gamma-1) Pr)
+    T invGm1;                         // 1 / (gamma-1)
+    __host__ __device__ static Eos make(const Params& P) {
+        Eos e;
+        e.gamma = (T)P.gamma;
+        e.gm1 = (T)(P.gamma - 1);
+        e.ScPlusOne = (T)P.ScPlusOne;
+        e.Re = (T)P.Re;
+        e.pf = (T)(1 / ((P.gamma - 1) * P.Pr));
+        e.invGm1 = (T)(1 / (P.gamma - 1));
+        return e;
+    }
+};
+
+// (1+T')^1.5 as x*sqrt(x); pow(x, 1.5) of the reference (src/Solver.cpp:476) differs by <= 1 ulp.
+__device__ __forceinline__ double pow15(double x) { return x * sqrt(x); }
+__device__ __forceinline__ float pow15(float x) { return x * sqrtf(x); }
+
+// Primitive variables and viscosity from conserved (updatePrimitiveVariables, src/Solver.cpp:446-479).
+template <class T>
+__device__ __forceinline__ void derivePrims(const Eos<T>& e, T rho, T ru, T rv, T rw, T rE, T& u, T& v, T& w, T& p,
+                                            T& Tt, T& mu) {
+    T m = rho + (T)1;
+    u = ru / m;
+    v = rv / m;
+    w = rw / m;
+    p = e.gm1 * (rE - m / (T)2 * (u * u + v * v + w * w));
+    Tt = (e.gamma * p - rho) / ((T)1 + rho);
+    mu = pow15((T)1 + Tt) * e.ScPlusOne / (e.Re * (Tt + e.ScPlusOne));
+}
+
+// Conserved from primitive (deriveConservedVariables, src/FlowVariableGroupStructs.h:152-160).
+template <class T>
+__device__ __forceinline__ void deriveCons(const Eos<T>& e, T u, T v, T w, T p, T Tt, T& rho, T& ru, T& rv, T& rw, T& rE) {
+    rho = (e.gamma * p - Tt) / ((T)1 + Tt);
+    T m = (T)1 + rho;
+    ru = m * u;
+    rv = m * v;
+    rw = m * w;
+    rE = p / e.gm1 + m / (T)2 * (u * u + v * v + w * w);
+}
+
+// Order-preserving encoding of a non-negative double for atomicMax on the bit pattern.
+__device__ __forceinline__ unsigned long long orderedBits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+}  // namespace nsfsi

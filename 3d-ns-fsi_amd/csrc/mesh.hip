@@ -1,0 +1,564 @@
+/*
+ * mesh.hip — node classification, ghost-node geometry and the per-stage boundary conditions.
+ *
+ * Everything in this file is compiled without FMA contraction (-ffp-contract=off on its command line and
+ * the pragma below), with IEEE division and square root, so that node classes, ghost sets, image points
+ * and the 8x8 factorisations come out bit-identical to a serial FP64 evaluation of the same expressions
+ * (the reference's g++ -O3 build emits no FMA on x86-64).
+ *
+ * Reference counterparts: Mesh::categorizeNodes (src/Mesh.cpp:61-82),
+ * MeshEdgeBoundary::identifyOwnedNodes (src/Boundary.cpp:18-71), the body classes
+ * (src/Boundary.cpp:461-615, 887-1103), the edge BCs (src/Boundary.cpp:142-417) and
+ * ImmersedBoundary::applyBoundaryCondition (src/Boundary.cpp:427-457, 618-883).
+ */
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace nsfsi {
+
+namespace {
+
+__device__ __forceinline__ void nodePos(const Params& P, int i, int j, int k, double& x, double& y, double& z) {
+    // getNodePosition, src/SmallVectors.h:56-63: (int index - offset) * spacing
+    x = ((double)i - P.ox) * P.hx;
+    y = ((double)j - P.oy) * P.hy;
+    z = ((double)k - P.oz) * P.hz;
+}
+
+// getSurroundingNodesBox, src/SmallVectors.h:108-127. Returns false if the box leaves the mesh.
+__device__ __forceinline__ bool surroundingBox(const Layout& L, const Params& P, double px, double py, double pz, int& i0,
+                                               int& j0, int& k0) {
+    double vx = ceil(px / P.hx) + P.ox, vy = ceil(py / P.hy) + P.oy, vz = ceil(pz / P.hz) + P.oz;
+    long in = vx >= 1.0 ? (long)vx : 1, jn = vy >= 1.0 ? (long)vy : 1, kn = vz >= 1.0 ? (long)vz : 1;
+    if (in > L.NI) in = L.NI;
+    if (jn > L.NJ) jn = L.NJ;
+    if (kn > L.NK) kn = L.NK;
+    i0 = (int)in - 1;
+    j0 = (int)jn - 1;
+    k0 = (int)kn - 1;
+    return in <= L.NI - 1 && jn <= L.NJ - 1 && kn <= L.NK - 1;
+}
+
+__device__ __forceinline__ double len3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
+
+// CylinderBody::getNormalProbe / SphereBody::getNormalProbe, src/Boundary.cpp:920-935, :1047-1053
+__device__ __forceinline__ void normalProbe(const BodyDesc& B, double gx, double gy, double gz, double& px, double& py,
+                                            double& pz) {
+    double cx = gx - B.cx, cy = gy - B.cy, cz = gz - B.cz;
+    if (B.type == NS_BODY_CYLINDER) {
+        if (B.axis == 0) cx = 0;
+        else if (B.axis == 1) cy = 0;
+        else cz = 0;
+    }
+    double l = len3(cx, cy, cz);
+    double lf = (B.r - l) / l;
+    px = cx * lf;
+    py = cy * lf;
+    pz = cz * lf;
+}
+
+__global__ void k_classify(Layout L, uint8_t* cls, int t0, int t1, int t2, int t3, int t4, int t5, BodyDesc B, Params P) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    int k = blockIdx.z;
+    if (i >= L.NI) return;
+    // Edge claims in push order z-min, z-max, y-min, y-max, x-min, x-max (src/Mesh.cpp:33-38, :67-69)
+    int t = NS_FLUID_INTERIOR;
+    if (k == 0) t = t4;
+    else if (k == L.NK - 1) t = t5;
+    else if (j == 0) t = t2;
+    else if (j == L.NJ - 1) t = t3;
+    else if (i == 0) t = t0;
+    else if (i == L.NI - 1) t = t1;
+    uint8_t flags = 0;
+    if (B.type != NS_BODY_NONE && B.box.contains(i, j, k)) {
+        double x, y, z;
+        nodePos(P, i, j, k, x, y, z);
+        double dx = x - B.cx, dy = y - B.cy, dz = z - B.cz, d;
+        if (B.type == NS_BODY_SPHERE) d = sqrt(dx * dx + dy * dy + dz * dz);
+        else if (B.axis == 0) d = sqrt(dy * dy + dz * dz);
+        else if (B.axis == 1) d = sqrt(dx * dx + dz * dz);
+        else d = sqrt(dx * dx + dy * dy);
+        // getSolidAndFilterNodesIn{Cylinder,Sphere}, src/Boundary.cpp:1005-1012, :1094-1101
+        if (d < B.r - B.mpb) t = NS_SOLID_INACTIVE;
+        else if (d < B.r + B.filterWidth && t == NS_FLUID_INTERIOR) flags |= CLS_FILTER;
+    }
+    cls[L.off(i, j, k)] = (uint8_t)(t | flags);
+}
+
+// findGhostNodesWithFluidNeighbors, src/Boundary.cpp:461-562: a solid node with a FluidInterior node
+// among its 18 face/edge neighbours becomes a ghost.
+__global__ void k_mark_ghosts(Layout L, uint8_t* cls, Window W) {
+    int i = W.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    int j = W.j0 + blockIdx.y;
+    int k = W.k0 + blockIdx.z;
+    if (i >= W.i0 + W.ni) return;
+    size_t o = L.off(i, j, k);
+    if ((cls[o] & CLS_TYPE_MASK) != NS_SOLID_INACTIVE) return;
+    bool has = false;
+    for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -1; dk <= 1; ++dk) {
+                int nz = (di != 0) + (dj != 0) + (dk != 0);
+                if (nz == 0 || nz == 3) continue;
+                int ii = i + di, jj = j + dj, kk = k + dk;
+                if (ii < 0 || jj < 0 || kk < 0 || ii >= L.NI || jj >= L.NJ || kk >= L.NK) continue;
+                if ((cls[L.off(ii, jj, kk)] & CLS_TYPE_MASK) == NS_FLUID_INTERIOR) has = true;
+            }
+    if (has) cls[o] = (uint8_t)((cls[o] & ~CLS_TYPE_MASK) | NS_SOLID_GHOST);
+}
+
+__device__ __forceinline__ void nodeIJK(const Layout& L, int64_t n, int& i, int& j, int& k) {
+    i = (int)(n % L.NI);
+    int64_t r = n / L.NI;
+    j = (int)(r % L.NJ);
+    k = (int)(r / L.NJ);
+}
+
+__global__ void k_count(Layout L, const uint8_t* cls, Window W, uint8_t mask, uint8_t value, int32_t* counts) {
+    __shared__ int32_t s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < W.size()) {
+        int i = W.i0 + (int)(w % W.ni);
+        size_t r = w / W.ni;
+        int j = W.j0 + (int)(r % W.nj);
+        int k = W.k0 + (int)(r / W.nj);
+        if ((cls[L.off(i, j, k)] & mask) == value) atomicAdd(&s, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = s;
+}
+
+__global__ void k_scatter(Layout L, const uint8_t* cls, Window W, uint8_t mask, uint8_t value, const int32_t* offsets,
+                          int64_t* out) {
+    __shared__ int32_t flags[256];
+    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int f = 0;
+    int64_t node = 0;
+    if (w < W.size()) {
+        int i = W.i0 + (int)(w % W.ni);
+        size_t r = w / W.ni;
+        int j = W.j0 + (int)(r % W.nj);
+        int k = W.k0 + (int)(r / W.nj);
+        f = (cls[L.off(i, j, k)] & mask) == value;
+        node = ((int64_t)k * L.NJ + j) * L.NI + i;
+    }
+    flags[threadIdx.x] = f;
+    __syncthreads();
+    if (f) {
+        int pos = 0;
+        for (int q = 0; q < (int)threadIdx.x; ++q) pos += flags[q];
+        out[offsets[blockIdx.x] + pos] = node;
+    }
+}
+
+// setImagePointPositions + checkIfSurroundingShouldBeGhost, src/Boundary.cpp:568-602
+__global__ void k_image_points(Layout L, uint8_t* cls, Params P, BodyDesc B, const int64_t* ghosts, int64_t n, double* bi,
+                               double* ip, int32_t* changed, int32_t* error) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int i, j, k;
+    nodeIJK(L, ghosts[g], i, j, k);
+    double gx, gy, gz, px, py, pz;
+    nodePos(P, i, j, k, gx, gy, gz);
+    normalProbe(B, gx, gy, gz, px, py, pz);
+    double bx = gx + px, by = gy + py, bz = gz + pz;
+    double qx = bx + px, qy = by + py, qz = bz + pz;
+    bi[3 * g] = bx; bi[3 * g + 1] = by; bi[3 * g + 2] = bz;
+    ip[3 * g] = qx; ip[3 * g + 1] = qy; ip[3 * g + 2] = qz;
+    int i0, j0, k0;
+    if (!surroundingBox(L, P, qx, qy, qz, i0, j0, k0)) { atomicOr(error, 1); return; }
+    for (int c = 0; c < 8; ++c) {
+        int ii = i0 + ((c >> 2) & 1), jj = j0 + ((c >> 1) & 1), kk = k0 + (c & 1);
+        size_t o = L.off(ii, jj, kk);
+        uint8_t v = cls[o];
+        if ((v & CLS_TYPE_MASK) == NS_SOLID_INACTIVE) {
+            cls[o] = (uint8_t)((v & ~CLS_TYPE_MASK) | NS_SOLID_GHOST);
+            atomicAdd(changed, 1);
+        }
+    }
+}
+
+__global__ void k_fill_gmap(Layout L, Window W, const int64_t* ghosts, int64_t n, int32_t* gmap) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int i, j, k;
+    nodeIJK(L, ghosts[g], i, j, k);
+    gmap[W.idx(i, j, k)] = (int32_t)g;
+}
+
+// Textbook Gaussian elimination with complete pivoting on an 8x8 row-major matrix (first maximum in a
+// column-major scan of the trailing block). Same algorithm and operation order as the serial FP64 path.
+__device__ void lu8Factor(double* A, int8_t* rp, int8_t* cp) {
+    for (int q = 0; q < 8; ++q) { rp[q] = (int8_t)q; cp[q] = (int8_t)q; }
+    for (int s = 0; s < 8; ++s) {
+        int pr = s, pc = s;
+        double best = -1;
+        for (int c = s; c < 8; ++c)
+            for (int r = s; r < 8; ++r) {
+                double m = fabs(A[r * 8 + c]);
+                if (m > best) { best = m; pr = r; pc = c; }
+            }
+        if (pr != s) {
+            for (int c = 0; c < 8; ++c) { double t = A[s * 8 + c]; A[s * 8 + c] = A[pr * 8 + c]; A[pr * 8 + c] = t; }
+            int8_t t = rp[s]; rp[s] = rp[pr]; rp[pr] = t;
+        }
+        if (pc != s) {
+            for (int r = 0; r < 8; ++r) { double t = A[r * 8 + s]; A[r * 8 + s] = A[r * 8 + pc]; A[r * 8 + pc] = t; }
+            int8_t t = cp[s]; cp[s] = cp[pc]; cp[pc] = t;
+        }
+        double piv = A[s * 8 + s];
+        if (piv == 0) continue;
+        for (int r = s + 1; r < 8; ++r) {
+            double l = A[r * 8 + s] / piv;
+            A[r * 8 + s] = l;
+            for (int c = s + 1; c < 8; ++c) A[r * 8 + c] -= l * A[s * 8 + c];
+        }
+    }
+}
+
+// Image-point geometry, corner tables and both Vandermonde factorisations for every ghost
+// (setInterpolationValues + populateVandermonde*, src/Boundary.cpp:655-728, :788-817).
+__global__ void k_ghost_geometry(Layout L, const uint8_t* cls, Params P, Window W, const int64_t* ghosts, int64_t n,
+                                 const int32_t* gmap, GhostTables G, int32_t* error) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int i, j, k;
+    nodeIJK(L, ghosts[g], i, j, k);
+    G.node[g] = (int64_t)L.off(i, j, k);
+    double qx = G.ip[3 * g], qy = G.ip[3 * g + 1], qz = G.ip[3 * g + 2];
+    int i0, j0, k0;
+    if (!surroundingBox(L, P, qx, qy, qz, i0, j0, k0)) { atomicOr(error, 1); return; }
+    G.boxLo[3 * g] = i0; G.boxLo[3 * g + 1] = j0; G.boxLo[3 * g + 2] = k0;
+    double X[8], Y[8], Z[8], nx[8], ny[8], nz[8];
+    uint8_t mask = 0;
+    for (int c = 0; c < 8; ++c) {
+        int ii = i0 + ((c >> 2) & 1), jj = j0 + ((c >> 1) & 1), kk = k0 + (c & 1);
+        size_t o = L.off(ii, jj, kk);
+        G.corner[8 * g + c] = (int64_t)o;
+        int t = cls[o] & CLS_TYPE_MASK;
+        nx[c] = ny[c] = nz[c] = 0;
+        if (t == NS_FLUID_INTERIOR || t == NS_FLUID_EDGE || t == NS_FLUID_GHOST) {
+            nodePos(P, ii, jj, kk, X[c], Y[c], Z[c]);
+        } else if (t == NS_SOLID_GHOST) {
+            mask |= (uint8_t)(1u << c);
+            int32_t og = W.contains(ii, jj, kk) ? gmap[W.idx(ii, jj, kk)] : -1;
+            if (og < 0) { atomicOr(error, 2); return; }
+            double bx = G.bi[3 * og], by = G.bi[3 * og + 1], bz = G.bi[3 * og + 2];
+            double vx = G.ip[3 * og] - bx, vy = G.ip[3 * og + 1] - by, vz = G.ip[3 * og + 2] - bz;
+            double l = len3(vx, vy, vz);
+            nx[c] = vx / l; ny[c] = vy / l; nz[c] = vz / l;
+            X[c] = bx; Y[c] = by; Z[c] = bz;
+        } else {
+            atomicOr(error, 4);  // "Impossible situation. Found solid node around image point."
+            return;
+        }
+    }
+    G.mask[g] = mask;
+    double lx, ly, lz;
+    nodePos(P, i0, j0, k0, lx, ly, lz);
+    G.rel[3 * g] = qx - lx; G.rel[3 * g + 1] = qy - ly; G.rel[3 * g + 2] = qz - lz;
+    if (mask == 0) return;
+    double* D = G.luD + 64 * g;
+    double* N = G.luN + 64 * g;
+    for (int r = 0; r < 8; ++r) {
+        double row[8] = {1, X[r], Y[r], Z[r], X[r] * Y[r], X[r] * Z[r], Y[r] * Z[r], X[r] * Y[r] * Z[r]};
+        for (int q = 0; q < 8; ++q) D[r * 8 + q] = N[r * 8 + q] = row[q];
+        if (mask & (1u << r)) {
+            N[r * 8 + 0] = 0;
+            N[r * 8 + 1] = nx[r];
+            N[r * 8 + 2] = ny[r];
+            N[r * 8 + 3] = nz[r];
+            N[r * 8 + 4] = X[r] * ny[r] + Y[r] * nx[r];
+            N[r * 8 + 5] = X[r] * nz[r] + Z[r] * nx[r];
+            N[r * 8 + 6] = Y[r] * nz[r] + Z[r] * ny[r];
+            N[r * 8 + 7] = X[r] * Y[r] * nz[r] + X[r] * Z[r] * ny[r] + Y[r] * Z[r] * nx[r];
+        }
+    }
+    int8_t* pm = G.perm + 32 * g;
+    lu8Factor(D, pm, pm + 8);
+    lu8Factor(N, pm + 16, pm + 24);
+}
+
+// ------------------------------------------------------------------------------------------------ BCs
+
+template <class T>
+struct Fields {
+    T* f[5];
+};
+template <class T>
+__device__ __forceinline__ Fields<T> fields(const Layout& L, T* S) {
+    Fields<T> F;
+    for (int q = 0; q < 5; ++q) F.f[q] = S + (size_t)q * L.fstride;
+    return F;
+}
+
+// One edge face, all owned nodes (src/Boundary.cpp:250-417). Each thread handles one node of the face;
+// the inlet's in-plane density filter recomputes its neighbours' extrapolated densities itself, so the
+// face needs no second pass.
+template <class T>
+__global__ void k_edge_face(Layout L, Params P, EdgeFace f, T* S, const double* tdev, const double* dtdev, double frac) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;  // first in-plane axis
+    int b = blockIdx.y;                             // second in-plane axis
+    // in-plane axes in the reference's loop order: x-normal (j,k), y-normal (i,k), z-normal (i,j)
+    int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
+    int lo[3] = {f.i0, f.j0, f.k0}, hi[3] = {f.i1, f.j1, f.k1};
+    int c[3];
+    c[f.axis] = lo[f.axis];
+    c[ax1] = lo[ax1] + a;
+    c[ax2] = lo[ax2] + b;
+    if (c[ax1] > hi[ax1] || c[ax2] > hi[ax2]) return;
+    Fields<T> F = fields(L, S);
+    Eos<T> e = Eos<T>::make(P);
+    size_t o = L.off(c[0], c[1], c[2]);
+    long st = f.axis == 0 ? 1 : (f.axis == 1 ? (long)L.pitch : (long)L.plane);
+    long s1 = f.side == 0 ? st : -st;  // towards the interior
+    if (f.type == NS_BC_INLET) {
+        auto extrap = [&](size_t oo) { return (T)2 * F.f[0][oo + s1] - F.f[0][oo + 2 * s1]; };
+        // filterInletDensity, src/Boundary.cpp:142-245: neighbours inside the owned box, order +a1,-a1,+a2,-a2
+        T acc = 0;
+        unsigned nb = 0;
+        long stA[3] = {1, (long)L.pitch, (long)L.plane};
+        for (int ax : {ax1, ax2}) {
+            if (c[ax] < hi[ax]) { acc += extrap(o + stA[ax]); ++nb; }
+            if (c[ax] > lo[ax]) { acc += extrap(o - stA[ax]); ++nb; }
+        }
+        acc += (T)nb * extrap(o);
+        acc /= (T)(2 * nb);
+        double ts = *tdev + frac * *dtdev;
+        T vel = (T)(fmin(1., ts / P.rampTime) * P.M_0);
+        if (f.side == 1) vel *= (T)-1;
+        T uu = 0, vv = 0, ww = 0;
+        if (f.axis == 0) uu = vel; else if (f.axis == 1) vv = vel; else ww = vel;
+        T Tt = 0;
+        T p = (Tt * ((T)1 + acc) + acc) / e.gamma;
+        T rho, ru, rv, rw, rE;
+        deriveCons(e, uu, vv, ww, p, Tt, rho, ru, rv, rw, rE);
+        F.f[0][o] = rho; F.f[1][o] = ru; F.f[2][o] = rv; F.f[3][o] = rw; F.f[4][o] = rE;
+    } else if (f.type == NS_BC_OUTLET) {
+        T r = (T)2 * F.f[0][o + s1] - F.f[0][o + 2 * s1];
+        T ru = (T)2 * F.f[1][o + s1] - F.f[1][o + 2 * s1];
+        T rv = (T)2 * F.f[2][o + s1] - F.f[2][o + 2 * s1];
+        T rw = (T)2 * F.f[3][o + s1] - F.f[3][o + 2 * s1];
+        T p = 0;
+        T u = ru / ((T)1 + r), v = rv / ((T)1 + r), w = rw / ((T)1 + r);
+        T rE = p / e.gm1 + ((T)1 + r) / (T)2 * (u * u + v * v + w * w);
+        F.f[0][o] = r; F.f[1][o] = ru; F.f[2][o] = rv; F.f[3][o] = rw; F.f[4][o] = rE;
+    } else if (f.type == NS_BC_PERIODIC) {
+        int cc[3] = {c[0], c[1], c[2]};
+        int Nn = f.axis == 0 ? L.NI : (f.axis == 1 ? L.NJ : L.NK);
+        cc[f.axis] = f.side == 0 ? Nn - 2 : 1;
+        size_t src = L.off(cc[0], cc[1], cc[2]);
+        for (int q = 0; q < 5; ++q) F.f[q][o] = F.f[q][src];
+    } else {  // symmetry: mirror the primitives of the node two in, negate the normal velocity
+        size_t a2 = o + 2 * s1;
+        T u, v, w, p, Tt, mu;
+        derivePrims(e, F.f[0][a2], F.f[1][a2], F.f[2][a2], F.f[3][a2], F.f[4][a2], u, v, w, p, Tt, mu);
+        if (f.axis == 0) u = -u; else if (f.axis == 1) v = -v; else w = -w;
+        T rho, ru, rv, rw, rE;
+        deriveCons(e, u, v, w, p, Tt, rho, ru, rv, rw, rE);
+        F.f[0][o] = rho; F.f[1][o] = ru; F.f[2][o] = rv; F.f[3][o] = rw; F.f[4][o] = rE;
+    }
+}
+
+// filterClosestFluidNodes, src/Boundary.cpp:618-652, first phase: filtered rho and rho*E into tmp, and
+// the pre-filter values into the side tables (window-dense), which the next stencil evaluation uses for
+// the primitive variables of filter nodes (the reference keeps them pre-filter).
+template <class T>
+__global__ void k_filter_compute(Layout L, Window W, const int64_t* fn, int64_t n, const T* S, T* tmp, T* rhoPre,
+                                 T* rhoEPre) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int i, j, k;
+    nodeIJK(L, fn[q], i, j, k);
+    size_t o = L.off(i, j, k);
+    const T* r = S;
+    const T* E = S + 4 * L.fstride;
+    const long sx = 1, sy = L.pitch, sz = (long)L.plane;
+    // order of the reference sum: P*6, i+1, i-1, j+1, j-1, k+1, k-1
+    T fr = ((T)6 * r[o] + r[o + sx] + r[o - sx] + r[o + sy] + r[o - sy] + r[o + sz] + r[o - sz]) / (T)12;
+    T fe = ((T)6 * E[o] + E[o + sx] + E[o - sx] + E[o + sy] + E[o - sy] + E[o + sz] + E[o - sz]) / (T)12;
+    tmp[2 * q] = fr;
+    tmp[2 * q + 1] = fe;
+    size_t w = W.idx(i, j, k);
+    rhoPre[w] = r[o];
+    rhoEPre[w] = E[o];
+}
+
+template <class T>
+__global__ void k_filter_write(Layout L, const int64_t* fn, int64_t n, T* S, const T* tmp) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int i, j, k;
+    nodeIJK(L, fn[q], i, j, k);
+    size_t o = L.off(i, j, k);
+    S[o] = tmp[2 * q];
+    S[o + 4 * L.fstride] = tmp[2 * q + 1];
+}
+
+__device__ __forceinline__ double luSolveDot(const double* LU, const int8_t* rp, const int8_t* cp, const double* f, double x,
+                                             double y, double z) {
+    double yv[8], zv[8], c[8];
+    for (int r = 0; r < 8; ++r) yv[r] = f[rp[r]];
+    for (int r = 1; r < 8; ++r)
+        for (int q = 0; q < r; ++q) yv[r] -= LU[r * 8 + q] * yv[q];
+    for (int r = 7; r >= 0; --r) {
+        double acc = yv[r];
+        for (int q = r + 1; q < 8; ++q) acc -= LU[r * 8 + q] * zv[q];
+        zv[r] = LU[r * 8 + r] != 0 ? acc / LU[r * 8 + r] : 0.0;
+    }
+    for (int r = 0; r < 8; ++r) c[cp[r]] = zv[r];
+    // trilinearInterpolation, src/Boundary.cpp:826-833
+    return c[0] + c[1] * x + c[2] * y + c[3] * z + c[4] * x * y + c[5] * x * z + c[6] * y * z + c[7] * x * y * z;
+}
+
+// Ghost update over the compacted ghost list (src/Boundary.cpp:432-456): gather the 8 image-point box
+// values, interpolate (tensor-product lerp when all 8 are fluid, else the stored FP64 Vandermonde
+// factors), mirror into the ghost. Interpolation runs in FP64 whatever the field precision.
+template <class T>
+__global__ void k_ghost_apply(Layout L, Params P, GhostTables G, int64_t n, T* S) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint8_t mask = G.mask[g];
+    double val[5][8];
+    for (int c = 0; c < 8; ++c) {
+        size_t o = (size_t)G.corner[8 * g + c];
+        bool gh = (mask >> c) & 1;
+        for (int q = 0; q < 5; ++q) val[q][c] = gh ? 0.0 : (double)S[o + (size_t)q * L.fstride];
+    }
+    double ipv[5];
+    if (mask == 0) {
+        double rx = G.rel[3 * g], ry = G.rel[3 * g + 1], rz = G.rel[3 * g + 2];
+        for (int q = 0; q < 5; ++q) {  // simplifiedInterpolation, src/Boundary.cpp:732-757
+            const double* f = val[q];
+            double a01 = f[0] + (f[1] - f[0]) * rz / P.hz;
+            double a23 = f[2] + (f[3] - f[2]) * rz / P.hz;
+            double a45 = f[4] + (f[5] - f[4]) * rz / P.hz;
+            double a67 = f[6] + (f[7] - f[6]) * rz / P.hz;
+            double a0123 = a01 + (a23 - a01) * ry / P.hy;
+            double a4567 = a45 + (a67 - a45) * ry / P.hy;
+            ipv[q] = a0123 + (a4567 - a0123) * rx / P.hx;
+        }
+    } else {
+        double x = G.ip[3 * g], y = G.ip[3 * g + 1], z = G.ip[3 * g + 2];
+        const double* D = G.luD + 64 * g;
+        const double* N = G.luN + 64 * g;
+        const int8_t* pm = G.perm + 32 * g;
+        ipv[0] = luSolveDot(N, pm + 16, pm + 24, val[0], x, y, z);
+        ipv[1] = luSolveDot(D, pm, pm + 8, val[1], x, y, z);
+        ipv[2] = luSolveDot(D, pm, pm + 8, val[2], x, y, z);
+        ipv[3] = luSolveDot(D, pm, pm + 8, val[3], x, y, z);
+        ipv[4] = luSolveDot(N, pm + 16, pm + 24, val[4], x, y, z);
+    }
+    size_t o = (size_t)G.node[g];
+    // getGhostNodeBCVariables, src/Boundary.cpp:777-785
+    S[o] = (T)ipv[0];
+    S[o + L.fstride] = (T)(-ipv[1]);
+    S[o + 2 * L.fstride] = (T)(-ipv[2]);
+    S[o + 3 * L.fstride] = (T)(-ipv[3]);
+    S[o + 4 * L.fstride] = (T)ipv[4];
+}
+
+inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------ launchers
+
+void launchClassify(hipStream_t s, const Layout& L, uint8_t* cls, const int ft[6], const BodyDesc& B, const Params& P) {
+    dim3 blk(128), grd(blocks(L.NI, 128), L.NJ, L.NK);
+    hipLaunchKernelGGL(k_classify, grd, blk, 0, s, L, cls, ft[0], ft[1], ft[2], ft[3], ft[4], ft[5], B, P);
+}
+
+void launchMarkGhosts(hipStream_t s, const Layout& L, uint8_t* cls, const Window& W) {
+    if (W.size() == 0) return;
+    dim3 blk(64), grd(blocks(W.ni, 64), W.nj, W.nk);
+    hipLaunchKernelGGL(k_mark_ghosts, grd, blk, 0, s, L, cls, W);
+}
+
+int64_t compactWindow(hipStream_t s, const Layout& L, const uint8_t* cls, const Window& W, uint8_t mask, uint8_t value,
+                      int64_t* out, int32_t* counts, int32_t* offsets) {
+    if (W.size() == 0) return 0;
+    unsigned nb = blocks((int64_t)W.size(), 256);
+    hipLaunchKernelGGL(k_count, dim3(nb), dim3(256), 0, s, L, cls, W, mask, value, counts);
+    std::vector<int32_t> h(nb);
+    (void)hipMemcpyAsync(h.data(), counts, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    int64_t total = 0;
+    for (unsigned b = 0; b < nb; ++b) {
+        int32_t c = h[b];
+        h[b] = (int32_t)total;
+        total += c;
+    }
+    (void)hipMemcpyAsync(offsets, h.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (out && total > 0) hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, L, cls, W, mask, value, offsets, out);
+    (void)hipStreamSynchronize(s);
+    return total;
+}
+
+void launchImagePoints(hipStream_t s, const Layout& L, uint8_t* cls, const Params& P, const BodyDesc& B,
+                       const int64_t* ghosts, int64_t n, double* bi, double* ip, int32_t* changed, int32_t* error) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_image_points, dim3(blocks(n, 128)), dim3(128), 0, s, L, cls, P, B, ghosts, n, bi, ip, changed, error);
+}
+
+void launchFillGhostMap(hipStream_t s, const Layout& L, const Window& W, const int64_t* ghosts, int64_t n, int32_t* gmap) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_gmap, dim3(blocks(n, 128)), dim3(128), 0, s, L, W, ghosts, n, gmap);
+}
+
+void launchGhostGeometry(hipStream_t s, const Layout& L, const uint8_t* cls, const Params& P, const Window& W,
+                         const int64_t* ghosts, int64_t n, const int32_t* gmap, GhostTables G, int32_t* error) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_ghost_geometry, dim3(blocks(n, 64)), dim3(64), 0, s, L, cls, P, W, ghosts, n, gmap, G, error);
+}
+
+template <class T>
+void launchEdgeFace(hipStream_t s, const Layout& L, const Params& P, const EdgeFace& f, T* S, const double* tdev,
+                    const double* dtdev, double frac) {
+    int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
+    int lo[3] = {f.i0, f.j0, f.k0}, hi[3] = {f.i1, f.j1, f.k1};
+    int n1 = hi[ax1] - lo[ax1] + 1, n2 = hi[ax2] - lo[ax2] + 1;
+    if (n1 <= 0 || n2 <= 0) return;
+    hipLaunchKernelGGL(k_edge_face<T>, dim3(blocks(n1, 64), n2), dim3(64), 0, s, L, P, f, S, tdev, dtdev, frac);
+}
+
+template <class T>
+void launchFilterCompute(hipStream_t s, const Layout& L, const Window& W, const int64_t* fn, int64_t n, const T* S, T* tmp,
+                         T* rhoPre, T* rhoEPre) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_filter_compute<T>, dim3(blocks(n, 128)), dim3(128), 0, s, L, W, fn, n, S, tmp, rhoPre, rhoEPre);
+}
+
+template <class T>
+void launchFilterWrite(hipStream_t s, const Layout& L, const int64_t* fn, int64_t n, T* S, const T* tmp) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_filter_write<T>, dim3(blocks(n, 128)), dim3(128), 0, s, L, fn, n, S, tmp);
+}
+
+template <class T>
+void launchGhostApply(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, T* S) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_ghost_apply<T>, dim3(blocks(n, 64)), dim3(64), 0, s, L, P, G, n, S);
+}
+
+#define NS_INST(T)                                                                                                           \
+    template void launchEdgeFace<T>(hipStream_t, const Layout&, const Params&, const EdgeFace&, T*, const double*,            \
+                                    const double*, double);                                                                   \
+    template void launchFilterCompute<T>(hipStream_t, const Layout&, const Window&, const int64_t*, int64_t, const T*, T*,   \
+                                         T*, T*);                                                                             \
+    template void launchFilterWrite<T>(hipStream_t, const Layout&, const int64_t*, int64_t, T*, const T*);                  \
+    template void launchGhostApply<T>(hipStream_t, const Layout&, const Params&, const GhostTables&, int64_t, T*);
+NS_INST(double)
+NS_INST(float)
+#undef NS_INST
+
+}  // namespace nsfsi

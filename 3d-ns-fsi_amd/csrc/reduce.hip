@@ -1,0 +1,198 @@
+/*
+ * reduce.hip — the FP64 scans that close a time step: norms of change (Mesh::getNormOfChange,
+ * src/Mesh.cpp:171-178), the finite check (Mesh::checkFinity, src/Mesh.h:48-61) and the two spectral
+ * maxima of the time-step limits (src/Solver.cpp:91-126), then the dt/t/timeLevel update
+ * (src/Solver.cpp:80-89, 164-166) on the device, so a step never waits for the host.
+ *
+ * Deterministic: fixed grid, per-block partials, one final block that combines them in a fixed order.
+ */
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "stage.h"
+
+namespace nsfsi {
+
+namespace {
+
+constexpr int RB = 256;  // threads per reduction block
+
+// partial slots: 0..3 maxima (x, y, z spectral radius, viscous factor), 4..8 sums of squared change,
+// 9 non-finite count
+constexpr int NP = 10;
+
+__device__ __forceinline__ double blockSum(double v, double* sh) {
+    v = wave_sum(v);
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) r += sh[q];
+    return r;
+}
+__device__ __forceinline__ double blockMax(double v, double* sh) {
+    v = wave_max(v);
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) r = fmax(r, sh[q]);
+    return r;
+}
+
+template <class T>
+__global__ void k_step_reduce(Layout L, Params P, const T* U, const T* A, const uint8_t* cls, Window W, const T* rhoPre,
+                              const T* rhoEPre, int withNorms, double* partials) {
+    __shared__ double sh[8];
+    const size_t F = L.fstride;
+    const size_t N = L.nodes();
+    Eos<T> e = Eos<T>::make(P);
+    const double vm = fmax(4. / 3, P.gamma / P.Pr);
+    double mx[4] = {0, 0, 0, 0}, sm[5] = {0, 0, 0, 0, 0}, bad = 0;
+    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(n % L.NI);
+        size_t r = n / L.NI;
+        int j = (int)(r % L.NJ), k = (int)(r / L.NJ);
+        size_t o = L.off(i, j, k);
+        uint8_t c = cls[o];
+        double a[5];
+        for (int q = 0; q < 5; ++q) a[q] = (double)A[o + q * F];
+        for (int q = 0; q < 5; ++q)
+            if (!isfinite(a[q])) bad += 1;
+        if (withNorms && (c & CLS_TYPE_MASK) == NS_FLUID_INTERIOR)
+            for (int q = 0; q < 5; ++q) {
+                double d = (double)U[o + q * F] - a[q];
+                sm[q] += d * d;
+            }
+        if (c & (CLS_DT_SKIP | CLS_INACTIVE_IC)) continue;
+        T rho = A[o], rE = A[o + 4 * F];
+        if (c & CLS_FILTER) {
+            size_t w = W.idx(i, j, k);
+            rho = rhoPre[w];
+            rE = rhoEPre[w];
+        }
+        T u, v, w, p, Tt, mu;
+        derivePrims(e, rho, A[o + F], A[o + 2 * F], A[o + 3 * F], rE, u, v, w, p, Tt, mu);
+        double cs = sqrt((1 + P.gamma * (double)p) / (1 + a[0]));
+        mx[0] = fmax(mx[0], cs + fabs((double)u));
+        mx[1] = fmax(mx[1], cs + fabs((double)v));
+        mx[2] = fmax(mx[2], cs + fabs((double)w));
+        mx[3] = fmax(mx[3], (double)mu / (a[0] + 1) * vm);
+    }
+    double* out = partials + (size_t)blockIdx.x * NP;
+    for (int q = 0; q < 4; ++q) {
+        double v = blockMax(mx[q], sh);
+        if (threadIdx.x == 0) out[q] = v;
+    }
+    for (int q = 0; q < 5; ++q) {
+        double v = blockSum(sm[q], sh);
+        if (threadIdx.x == 0) out[4 + q] = v;
+    }
+    double b = blockSum(bad, sh);
+    if (threadIdx.x == 0) out[9] = b;
+}
+
+// SolidInactive nodes keep their initial primitive values (never written by the reference); their
+// density is whatever the result buffer holds there.
+template <class T>
+__global__ void k_inactive_reduce(Layout L, Params P, const T* A, const int64_t* nodes, const double* icPrim, int64_t n,
+                                  double* partial) {
+    __shared__ double sh[8];
+    const double vm = fmax(4. / 3, P.gamma / P.Pr);
+    double mx[4] = {0, 0, 0, 0};
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+        int64_t m = nodes[q];
+        int i = (int)(m % L.NI);
+        int64_t r = m / L.NI;
+        int j = (int)(r % L.NJ), k = (int)(r / L.NJ);
+        double rho = (double)A[L.off(i, j, k)];
+        const double* ic = icPrim + 5 * q;  // u, v, w, p, mu
+        double cs = sqrt((1 + P.gamma * ic[3]) / (1 + rho));
+        mx[0] = fmax(mx[0], cs + fabs(ic[0]));
+        mx[1] = fmax(mx[1], cs + fabs(ic[1]));
+        mx[2] = fmax(mx[2], cs + fabs(ic[2]));
+        mx[3] = fmax(mx[3], ic[4] / (rho + 1) * vm);
+    }
+    for (int q = 0; q < 4; ++q) {
+        double v = blockMax(mx[q], sh);
+        if (threadIdx.x == 0) partial[q] = v;
+    }
+    if (threadIdx.x == 0)
+        for (int q = 4; q < NP; ++q) partial[q] = 0;
+}
+
+// Combines the partials and advances t, timeLevel, dt. mode 0: initial dt only (Solver::initialize);
+// mode 1: end of a step.
+__global__ void k_finalize(Params P, const double* partials, int nPartials, int mode, int convHistory, int stopEndTime,
+                           double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap) {
+    __shared__ double sh[8];
+    double v[NP];
+    for (int q = 0; q < NP; ++q) v[q] = 0;
+    for (int b = threadIdx.x; b < nPartials; b += blockDim.x) {
+        const double* p = partials + (size_t)b * NP;
+        for (int q = 0; q < 4; ++q) v[q] = fmax(v[q], p[q]);
+        for (int q = 4; q < NP; ++q) v[q] += p[q];
+    }
+    double red[NP];
+    for (int q = 0; q < 4; ++q) red[q] = blockMax(v[q], sh);
+    for (int q = 4; q < NP; ++q) red[q] = blockSum(v[q], sh);
+    if (threadIdx.x != 0) return;
+    if (red[9] > 0) st->nonFinite = 1;
+    double t = st->t;
+    if (mode == 1) {
+        if (convHistory > 0) {
+            int slot = (int)(st->histCount % (uint64_t)histCap);
+            double vol = P.hx * P.hy * P.hz;
+            for (int q = 0; q < 5; ++q) hist[5 * slot + q] = sqrt(vol * red[4 + q]);
+            st->histCount += 1;
+        }
+        t = t + st->dt;
+        st->t = t;
+        st->level += 1;
+    }
+    // updateTimeStepSize, src/Solver.cpp:80-126
+    double dtConv = fabs(cfl) / (red[0] / P.hx + red[1] / P.hy + red[2] / P.hz);
+    double dx2 = P.hx * P.hx, dy2 = P.hy * P.hy, dz2 = P.hz * P.hz;
+    double dtVisc = fabs(sigma) / (red[3] * (1 / dx2 + 1 / dy2 + 1 / dz2));
+    double dt = fmin(dtConv, dtVisc);
+    if (stopEndTime && t + dt > tEnd) dt = tEnd - t;
+    st->dt = dt;
+}
+
+}  // namespace
+
+template <class T>
+void launchStepReduce(hipStream_t s, const Layout& L, const Params& P, const T* U, const T* A, const uint8_t* cls,
+                      const Window& W, const T* rhoPre, const T* rhoEPre, int withNorms, double* partials, int nBlocks) {
+    hipLaunchKernelGGL(k_step_reduce<T>, dim3(nBlocks), dim3(RB), 0, s, L, P, U, A, cls, W, rhoPre, rhoEPre, withNorms,
+                       partials);
+}
+
+template <class T>
+void launchInactiveReduce(hipStream_t s, const Layout& L, const Params& P, const T* A, const int64_t* nodes,
+                          const double* icPrim, int64_t n, double* partial) {
+    hipLaunchKernelGGL(k_inactive_reduce<T>, dim3(1), dim3(RB), 0, s, L, P, A, nodes, icPrim, n, partial);
+}
+
+void launchFinalize(hipStream_t s, const Params& P, const double* partials, int nPartials, int mode, int convHistory,
+                    int stopEndTime, double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RB), 0, s, P, partials, nPartials, mode, convHistory, stopEndTime, tEnd,
+                       cfl, sigma, st, hist, histCap);
+}
+
+int reducePartialSlots() { return NP; }
+
+template void launchStepReduce<double>(hipStream_t, const Layout&, const Params&, const double*, const double*,
+                                       const uint8_t*, const Window&, const double*, const double*, int, double*, int);
+template void launchStepReduce<float>(hipStream_t, const Layout&, const Params&, const float*, const float*,
+                                      const uint8_t*, const Window&, const float*, const float*, int, double*, int);
+template void launchInactiveReduce<double>(hipStream_t, const Layout&, const Params&, const double*, const int64_t*,
+                                           const double*, int64_t, double*);
+template void launchInactiveReduce<float>(hipStream_t, const Layout&, const Params&, const float*, const int64_t*,
+                                          const double*, int64_t, double*);
+
+}  // namespace nsfsi

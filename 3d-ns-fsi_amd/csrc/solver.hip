@@ -1,0 +1,698 @@
+/*
+ * solver.hip — host orchestration of one case on one GPU: the Mesh (classification, boundaries, device
+ * fields) and the Solver (RK4 stages, boundary applications, dt), mirroring the reference's
+ * Mesh / Solver split (src/Mesh.cpp, src/Solver.cpp). Kernels are launched on one HIP stream; a time
+ * step never synchronises with the host.
+ */
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "case.h"
+#include "device_common.h"
+#include "kernels.h"
+#include "stage.h"
+
+namespace nsfsi {
+
+// ------------------------------------------------------------------------------------------------ errors
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Layout makeLayout(int NI, int NJ, int NK, size_t eb) {
+    Layout L;
+    L.NI = NI;
+    L.NJ = NJ;
+    L.NK = NK;
+    size_t rowElems = 128 / eb;
+    L.pitch = (int)(((size_t)NI + rowElems - 1) / rowElems * rowElems);
+    L.plane = (size_t)NJ * L.pitch;
+    size_t fe = 256 / eb;
+    L.fstride = ((size_t)NK * L.plane + fe - 1) / fe * fe;
+    return L;
+}
+
+namespace {
+
+__global__ void k_flag_inactive(Layout L, uint8_t* cls, Window W, int perX, int perY, int perZ) {
+    int i = W.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    int j = W.j0 + blockIdx.y, k = W.k0 + blockIdx.z;
+    if (i >= W.i0 + W.ni) return;
+    size_t o = L.off(i, j, k);
+    if ((cls[o] & CLS_TYPE_MASK) != NS_SOLID_INACTIVE) return;
+    bool onX = i == 0 || i == L.NI - 1, onY = j == 0 || j == L.NJ - 1, onZ = k == 0 || k == L.NK - 1;
+    if (!onX && !onY && !onZ) { cls[o] |= CLS_INACTIVE_IC; return; }
+    // A node on a periodic plane receives a copy of its source node; when that source is itself an
+    // untouched inactive node, the copy adds nothing new to the dt maxima.
+    int si = i, sj = j, sk = k;
+    bool periodicOwner = false;
+    if (onZ && perZ) { sk = k == 0 ? L.NK - 2 : 1; periodicOwner = true; }
+    else if (!onZ && onY && perY) { sj = j == 0 ? L.NJ - 2 : 1; periodicOwner = true; }
+    else if (!onZ && !onY && onX && perX) { si = i == 0 ? L.NI - 2 : 1; periodicOwner = true; }
+    if (!periodicOwner) return;
+    bool sOnX = si == 0 || si == L.NI - 1, sOnY = sj == 0 || sj == L.NJ - 1, sOnZ = sk == 0 || sk == L.NK - 1;
+    if ((cls[L.off(si, sj, sk)] & CLS_TYPE_MASK) == NS_SOLID_INACTIVE && !sOnX && !sOnY && !sOnZ) cls[o] |= CLS_DT_SKIP;
+}
+
+template <class T>
+__global__ void k_gather_ic(Layout L, Params P, const T* U, const int64_t* nodes, int64_t n, double* ic) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int64_t m = nodes[q];
+    int i = (int)(m % L.NI);
+    int64_t r = m / L.NI;
+    int j = (int)(r % L.NJ), k = (int)(r / L.NJ);
+    size_t o = L.off(i, j, k);
+    Eos<T> e = Eos<T>::make(P);
+    T u, v, w, p, Tt, mu;
+    derivePrims(e, U[o], U[o + L.fstride], U[o + 2 * L.fstride], U[o + 3 * L.fstride], U[o + 4 * L.fstride], u, v, w, p, Tt,
+                mu);
+    ic[5 * q] = u; ic[5 * q + 1] = v; ic[5 * q + 2] = w; ic[5 * q + 3] = p; ic[5 * q + 4] = mu;
+}
+
+// Primitive output for SolidInactive nodes: their initial values (never updated by the reference).
+template <class T>
+__global__ void k_patch_inactive(Layout L, Params P, T* prim, const int64_t* nodes, const double* ic, int64_t n) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int64_t m = nodes[q];
+    int i = (int)(m % L.NI);
+    int64_t r = m / L.NI;
+    int j = (int)(r % L.NJ), k = (int)(r / L.NJ);
+    size_t o = L.off(i, j, k);
+    const double* c = ic + 5 * q;
+    prim[o] = (T)c[0];
+    prim[o + L.fstride] = (T)c[1];
+    prim[o + 2 * L.fstride] = (T)c[2];
+    prim[o + 3 * L.fstride] = (T)c[3];
+    // T' from the initial state: T' = (gamma p' - rho')/(1+rho') with rho' = 0 for every built-in IC
+    prim[o + 5 * L.fstride] = (T)c[4];
+    prim[o + 6 * L.fstride] = (T)(c[4] * (1 / ((P.gamma - 1) * P.Pr)));
+}
+
+template <class T>
+__global__ void k_totals(Layout L, const T* S, const uint8_t* cls, double* partials) {
+    __shared__ double sh[8];
+    const size_t N = L.nodes();
+    double sm[5] = {0, 0, 0, 0, 0};
+    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(n % L.NI);
+        size_t r = n / L.NI;
+        int j = (int)(r % L.NJ), k = (int)(r / L.NJ);
+        size_t o = L.off(i, j, k);
+        if ((cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) continue;
+        for (int q = 0; q < 5; ++q) sm[q] += (double)S[o + q * L.fstride];
+    }
+    for (int q = 0; q < 5; ++q) {
+        double v = wave_sum(sm[q]);
+        int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        __syncthreads();
+        if (lane == 0) sh[wv] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 0;
+            for (int b = 0; b < (int)(blockDim.x >> 6); ++b) r += sh[b];
+            partials[(size_t)blockIdx.x * 5 + q] = r;
+        }
+    }
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------ Case
+
+Case::Case(const ns_config& c) : cfg(c) {
+    setDerived(cfg);
+    std::string v = validateConfig(cfg);
+    if (!v.empty()) throw ConfigError(v);
+    fp32 = cfg.precision == NS_FP32;
+    eb = fp32 ? sizeof(float) : sizeof(double);
+    int dev = 0;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw HipError("no HIP device");
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        throw HipError(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+    hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    L = makeLayout(cfg.NI, cfg.NJ, cfg.NK, eb);
+}
+
+Case::~Case() {
+    for (void* p : allocations) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void* Case::alloc(size_t bytes) {
+    void* p = nullptr;
+    hipCheck(hipMalloc(&p, std::max<size_t>(bytes, 256)), "hipMalloc");
+    allocations.push_back(p);
+    return p;
+}
+
+// Mesh::setupBoundaries, src/Mesh.cpp:25-58, with the face types and body of the configuration.
+void Case::setupBoundaries() {
+    double Ls[3] = {cfg.L_x, cfg.L_y, cfg.L_z};
+    int Ns[3] = {cfg.NI, cfg.NJ, cfg.NK};
+    double h[3], off[3];
+    for (int a = 0; a < 3; ++a) {
+        int tmin = cfg.bc[2 * a], tmax = cfg.bc[2 * a + 1];
+        int gmin = (tmin == NS_BC_PERIODIC || tmin == NS_BC_SYMMETRY) ? 1 : 0;
+        int gmax = tmax == NS_BC_SYMMETRY ? 1 : 0;
+        h[a] = Ls[a] / (double)(Ns[a] - 1 - gmin - gmax);
+        off[a] = gmin;
+    }
+    P.hx = h[0]; P.hy = h[1]; P.hz = h[2];
+    P.ox = off[0]; P.oy = off[1]; P.oz = off[2];
+    P.gamma = cfg.gamma;
+    P.Pr = cfg.Pr;
+    P.Re = cfg.Re;
+    P.ScPlusOne = 1 + cfg.sutherlands_C2 / cfg.T_0;
+    P.M_0 = cfg.M_0;
+    P.rampTime = cfg.inlet_ramp_time;
+    // identifyOwnedNodes in push order z-min, z-max, y-min, y-max, x-min, x-max; applied in reverse
+    int ulo[3] = {0, 0, 0}, uhi[3] = {cfg.NI - 1, cfg.NJ - 1, cfg.NK - 1};
+    const int order[6][2] = {{2, 0}, {2, 1}, {1, 0}, {1, 1}, {0, 0}, {0, 1}};
+    EdgeFace pushed[6];
+    for (int q = 0; q < 6; ++q) {
+        int a = order[q][0], s = order[q][1];
+        EdgeFace f;
+        f.axis = a;
+        f.side = s;
+        f.type = cfg.bc[2 * a + s];
+        int lo[3] = {ulo[0], ulo[1], ulo[2]}, hi[3] = {uhi[0], uhi[1], uhi[2]};
+        if (s == 0) { hi[a] = lo[a]; ulo[a]++; }
+        else { lo[a] = hi[a]; uhi[a]--; }
+        f.i0 = lo[0]; f.i1 = hi[0]; f.j0 = lo[1]; f.j1 = hi[1]; f.k0 = lo[2]; f.k1 = hi[2];
+        pushed[q] = f;
+    }
+    for (int q = 0; q < 6; ++q) faces[q] = pushed[5 - q];
+    // body
+    B = BodyDesc{};
+    B.type = cfg.body_type;
+    B.axis = cfg.body_axis;
+    B.cx = cfg.body_centre[0]; B.cy = cfg.body_centre[1]; B.cz = cfg.body_centre[2];
+    B.r = cfg.body_radius;
+    B.mpb = cfg.machine_precision_buffer;
+    W = Window{0, 0, 0, 0, 0, 0};
+    if (B.type != NS_BODY_NONE) {
+        const int layer = 2;
+        if (B.type == NS_BODY_SPHERE) B.filterWidth = layer * std::max({h[0], h[1], h[2]});
+        else if (B.axis == 0) B.filterWidth = layer * std::max(h[1], h[2]);
+        else if (B.axis == 1) B.filterWidth = layer * std::max(h[0], h[2]);
+        else B.filterWidth = layer * std::max(h[0], h[1]);
+        // getSphereBoundingBox / getCylinderBoundingBox, src/Boundary.cpp:938-965, :1056-1072: centre index
+        // round(c/h) without the origin offset, radius ceil(r/h) + 1 + layer
+        double c[3] = {B.cx, B.cy, B.cz};
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            if (B.type == NS_BODY_CYLINDER && a == B.axis) { lo[a] = 0; hi[a] = Ns[a] - 1; continue; }
+            long rad = (long)std::ceil(B.r / h[a]) + 1 + layer;
+            double rc = std::round(c[a] / h[a]);
+            if (!(rc >= 0)) throw GeometryError("body bounding box below index 0");
+            long ci = (long)rc;
+            if (ci - rad < 0 || ci + rad > Ns[a] - 1) throw GeometryError("body bounding box leaves the mesh");
+            lo[a] = (int)(ci - rad);
+            hi[a] = (int)(ci + rad);
+        }
+        W = Window{lo[0], lo[1], lo[2], hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+        B.box = W;
+    }
+}
+
+// Mesh::categorizeNodes, src/Mesh.cpp:61-82, on the device.
+void Case::categorizeNodes() {
+    cls = (uint8_t*)alloc((size_t)L.NK * L.plane);
+    hipCheck(hipMemsetAsync(cls, 0, (size_t)L.NK * L.plane, stream), "memset cls");
+    int ft[6];
+    for (int q = 0; q < 6; ++q) ft[q] = (cfg.bc[q] == NS_BC_INLET || cfg.bc[q] == NS_BC_OUTLET) ? NS_FLUID_EDGE : NS_FLUID_GHOST;
+    launchClassify(stream, L, cls, ft, B, P);
+    nGhost = nFilter = nInactive = 0;
+    if (B.type == NS_BODY_NONE) return;
+    launchMarkGhosts(stream, L, cls, W);
+    size_t nb = (W.size() + 255) / 256;
+    int32_t* counts = (int32_t*)alloc(nb * sizeof(int32_t));
+    int32_t* offsets = (int32_t*)alloc(nb * sizeof(int32_t));
+    int32_t* flags = (int32_t*)alloc(2 * sizeof(int32_t));  // changed, error
+    int64_t* list = (int64_t*)alloc(W.size() * sizeof(int64_t));
+    double* bi = (double*)alloc(3 * W.size() * sizeof(double));
+    double* ip = (double*)alloc(3 * W.size() * sizeof(double));
+    // iterative ghost augmentation (CylinderBody::identifyRelatedNodes, src/Boundary.cpp:907-916): repeat
+    // until no SolidInactive node remains in any image-point box
+    for (int round = 0;; ++round) {
+        int64_t n = compactWindow(stream, L, cls, W, CLS_TYPE_MASK, NS_SOLID_GHOST, list, counts, offsets);
+        hipCheck(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), stream), "memset flags");
+        launchImagePoints(stream, L, cls, P, B, list, n, bi, ip, flags, flags + 1);
+        int32_t h[2];
+        hipCheck(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, stream), "copy flags");
+        hipCheck(hipStreamSynchronize(stream), "sync");
+        if (h[1]) throw GeometryError("image point box leaves the mesh");
+        if (h[0] == 0) { nGhost = n; break; }
+        if (round > 1000) throw GeometryError("ghost augmentation does not converge");
+    }
+    ghostList = list;
+    int32_t* gmap = (int32_t*)alloc(W.size() * sizeof(int32_t));
+    hipCheck(hipMemsetAsync(gmap, 0xff, W.size() * sizeof(int32_t), stream), "memset gmap");
+    launchFillGhostMap(stream, L, W, ghostList, nGhost, gmap);
+    G.node = (int64_t*)alloc(nGhost * sizeof(int64_t));
+    G.corner = (int64_t*)alloc(8 * nGhost * sizeof(int64_t));
+    G.mask = (uint8_t*)alloc(nGhost);
+    G.rel = (double*)alloc(3 * nGhost * sizeof(double));
+    G.ip = ip;
+    G.bi = bi;
+    G.luD = (double*)alloc(64 * nGhost * sizeof(double));
+    G.luN = (double*)alloc(64 * nGhost * sizeof(double));
+    G.perm = (int8_t*)alloc(32 * nGhost);
+    G.boxLo = (int32_t*)alloc(3 * nGhost * sizeof(int32_t));
+    hipCheck(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), stream), "memset flags");
+    launchGhostGeometry(stream, L, cls, P, W, ghostList, nGhost, gmap, G, flags + 1);
+    int32_t h[2];
+    hipCheck(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, stream), "copy flags");
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    if (h[1] & 4) throw GeometryError("Impossible situation. Found solid node around image point.");
+    if (h[1]) throw GeometryError("ghost geometry error");
+    // filter nodes (ascending)
+    filterList = (int64_t*)alloc(std::max<int64_t>(1, (int64_t)W.size()) * sizeof(int64_t));
+    nFilter = compactWindow(stream, L, cls, W, CLS_FILTER, CLS_FILTER, filterList, counts, offsets);
+    // SolidInactive bookkeeping for the dt scan
+    dim3 blk(64), grd(nblk(W.ni, 64), W.nj, W.nk);
+    hipLaunchKernelGGL(k_flag_inactive, grd, blk, 0, stream, L, cls, W, cfg.bc[NS_XMIN] == NS_BC_PERIODIC,
+                       cfg.bc[NS_YMIN] == NS_BC_PERIODIC, cfg.bc[NS_ZMIN] == NS_BC_PERIODIC);
+    inactiveList = (int64_t*)alloc(W.size() * sizeof(int64_t));
+    nInactive = compactWindow(stream, L, cls, W, CLS_INACTIVE_IC, CLS_INACTIVE_IC, inactiveList, counts, offsets);
+    inactiveIC = (double*)alloc(5 * std::max<int64_t>(1, nInactive) * sizeof(double));
+}
+
+template <class T>
+void Case::allocFields() {
+    size_t fb = 5 * L.fstride * sizeof(T);
+    U = alloc(fb);
+    A = alloc(fb);
+    S1 = alloc(fb);
+    S2 = alloc(fb);
+    prim = alloc(7 * L.fstride * sizeof(T));
+    hipCheck(hipMemsetAsync(U, 0, fb, stream), "memset");
+    hipCheck(hipMemsetAsync(A, 0, fb, stream), "memset");  // the reference's zero-initialised "Old" arrays
+    hipCheck(hipMemsetAsync(S1, 0, fb, stream), "memset");
+    hipCheck(hipMemsetAsync(S2, 0, fb, stream), "memset");
+    size_t wn = std::max<size_t>(1, W.size());
+    rhoPre = alloc(wn * sizeof(T));
+    rhoEPre = alloc(wn * sizeof(T));
+    ftmp = alloc(2 * std::max<int64_t>(1, nFilter) * sizeof(T));
+    nPartials = 1024;
+    partials = (double*)alloc((size_t)(nPartials + 1) * reducePartialSlots() * sizeof(double));
+    scal = (StepScalars*)alloc(sizeof(StepScalars));
+    hist = (double*)alloc((size_t)histCap * 5 * sizeof(double));
+    StepScalars s0{};
+    hipCheck(hipMemcpyAsync(scal, &s0, sizeof(s0), hipMemcpyHostToDevice, stream), "copy scalars");
+}
+
+// Initial condition on the host (applyStagnation_IC / applyUniformFlow_IC, src/Solver.cpp:31-75, or
+// freestream plus a counter-based seeded perturbation), in the device layout.
+template <class T>
+void Case::hostInitialField(std::vector<T>& h) const {
+    h.assign(5 * L.fstride, (T)0);
+    const double g = cfg.gamma;
+    double u0 = 0, v0 = 0, w0 = 0;
+    bool perturb = false;
+    if (cfg.ic_type == NS_IC_UNIFORM_OBLIQUE) { u0 = v0 = w0 = cfg.M_0 / std::sqrt(3); }
+    else if (cfg.ic_type == NS_IC_FREESTREAM_GAUSSIAN || cfg.ic_type == NS_IC_FREESTREAM_UNIFORM) { u0 = cfg.M_0; perturb = true; }
+    const bool gauss = cfg.ic_type == NS_IC_FREESTREAM_GAUSSIAN;
+    const double a = cfg.ic_amplitude;
+    const uint64_t seed = cfg.ic_seed;
+    auto mix = [](uint64_t z) {
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        return z ^ (z >> 31);
+    };
+    auto work = [&](int k0, int k1) {
+        for (int k = k0; k < k1; ++k)
+            for (int j = 0; j < L.NJ; ++j)
+                for (int i = 0; i < L.NI; ++i) {
+                    double u = u0, v = v0, w = w0, p = 0, Tt = 0;
+                    if (perturb) {
+                        uint64_t base = (((uint64_t)k * L.NJ + j) * L.NI + i) * 6;
+                        double d[3];
+                        for (int c = 0; c < 3; ++c) {
+                            double r1 = (double)(mix(seed + 0x9E3779B97F4A7C15ULL * (base + 2 * c + 1)) >> 11) * 0x1.0p-53;
+                            double r2 = (double)(mix(seed + 0x9E3779B97F4A7C15ULL * (base + 2 * c + 2)) >> 11) * 0x1.0p-53;
+                            d[c] = gauss ? a * (std::sqrt(-2.0 * std::log(1.0 - r1)) * std::cos(6.283185307179586 * r2))
+                                         : a * (2.0 * r1 - 1.0);
+                        }
+                        u = u0 + d[0];
+                        v = v0 + d[1];
+                        w = w0 + d[2];
+                    }
+                    double rho = (g * p - Tt) / (1 + Tt);
+                    double m = 1 + rho;
+                    double E = p / (g - 1) + m / 2 * (u * u + v * v + w * w);
+                    size_t o = L.off(i, j, k);
+                    h[o] = (T)rho;
+                    h[o + L.fstride] = (T)(m * u);
+                    h[o + 2 * L.fstride] = (T)(m * v);
+                    h[o + 3 * L.fstride] = (T)(m * w);
+                    h[o + 4 * L.fstride] = (T)E;
+                }
+    };
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if ((size_t)L.NK * L.NJ * L.NI < (1u << 20)) nt = 1;
+    std::vector<std::thread> th;
+    int chunk = (L.NK + (int)nt - 1) / (int)nt;
+    for (unsigned q = 0; q < nt; ++q) {
+        int k0 = (int)q * chunk, k1 = std::min(L.NK, k0 + chunk);
+        if (k0 < k1) th.emplace_back(work, k0, k1);
+    }
+    for (auto& t : th) t.join();
+}
+
+template <class T>
+void Case::uploadState(const std::vector<T>& h) {
+    hipCheck(hipMemcpyAsync(U, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream), "upload IC");
+    // pre-filter side tables start as the current values (the reference's primitives match the IC)
+    launchFilterCompute<T>(stream, L, W, filterList, nFilter, (const T*)U, (T*)ftmp, (T*)rhoPre, (T*)rhoEPre);
+    if (nInactive > 0)
+        hipLaunchKernelGGL(k_gather_ic<T>, dim3(nblk(nInactive, 128)), dim3(128), 0, stream, L, P, (const T*)U,
+                           inactiveList, nInactive, inactiveIC);
+    // updateTimeStepSize(0) on the initial state
+    computeDt<T>(0);
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    hipCheck(hipGetLastError(), "init kernels");
+}
+
+template <class T>
+void Case::computeDt(int mode) {
+    launchStepReduce<T>(stream, L, P, (const T*)U, mode == 0 ? (const T*)U : (const T*)A, cls, W, (const T*)rhoPre,
+                        (const T*)rhoEPre, mode, partials, nPartials);
+    int npart = nPartials;
+    if (nInactive > 0) {
+        launchInactiveReduce<T>(stream, L, P, mode == 0 ? (const T*)U : (const T*)A, inactiveList, inactiveIC, nInactive,
+                                partials + (size_t)nPartials * reducePartialSlots());
+        npart += 1;
+    }
+    launchFinalize(stream, P, partials, npart, mode, cfg.conv_history, cfg.stop_criterion == 1, cfg.t_end,
+                   cfg.conv_stability_limit, cfg.visc_stability_limit, scal, hist, histCap);
+}
+
+template <class T>
+void Case::applyBCs(T* S, double frac) {
+    // Mesh::applyAllBoundaryConditions, src/Mesh.cpp:155-167: edge faces in reverse claim order, then the
+    // immersed body (filter first, then ghosts; src/Boundary.cpp:431-456)
+    const double* tdev = &scal->t;
+    const double* dtdev = &scal->dt;
+    for (int q = 0; q < 6; ++q) launchEdgeFace<T>(stream, L, P, faces[q], S, tdev, dtdev, frac);
+    if (B.type == NS_BODY_NONE) return;
+    launchFilterCompute<T>(stream, L, W, filterList, nFilter, S, (T*)ftmp, (T*)rhoPre, (T*)rhoEPre);
+    launchFilterWrite<T>(stream, L, filterList, nFilter, S, (const T*)ftmp);
+    launchGhostApply<T>(stream, L, P, G, nGhost, S);
+}
+
+// Solver::marchTimeStep, src/Solver.cpp:130-167 (without the rho filter, see ns_init)
+template <class T>
+void Case::stepOnce() {
+    T* u = (T*)U;
+    T* a = (T*)A;
+    T* s1 = (T*)S1;
+    T* s2 = (T*)S2;
+    T* pr = (T*)prim;
+    const double* dtdev = &scal->dt;
+    T* in[4] = {u, s1, s2, s1};
+    T* out[4] = {s1, s2, s1, a};
+    const double frac[4] = {0.5, 0.5, 1.0, 1.0};
+    for (int st = 1; st <= 4; ++st) {
+        mark(1, true);
+        launchDerive<T>(stream, L, P, in[st - 1], pr, cls, W, (const T*)rhoPre, (const T*)rhoEPre);
+        mark(1, false);
+        mark(0, true);
+        launchRhsPlain<T>(stream, st, L, P, in[st - 1], pr, u, out[st - 1], a, cls, dtdev);
+        mark(0, false);
+        mark(2, true);
+        applyBCs<T>(out[st - 1], frac[st - 1]);
+        mark(2, false);
+    }
+    mark(3, true);
+    computeDt<T>(1);
+    mark(3, false);
+    std::swap(U, A);
+}
+
+void Case::mark(int group, bool begin) {
+    if (!profiling) return;
+    if (begin) {
+        EventPair e{};
+        hipCheck(hipEventCreate(&e.a), "event");
+        hipCheck(hipEventCreate(&e.b), "event");
+        e.group = group;
+        hipCheck(hipEventRecord(e.a, stream), "record");
+        events.push_back(e);
+    } else {
+        for (auto it = events.rbegin(); it != events.rend(); ++it)
+            if (it->group == group) { hipCheck(hipEventRecord(it->b, stream), "record"); break; }
+    }
+}
+
+void Case::profile(int nsteps, double ms[4], int launches[4]) {
+    if (!initialized) throw StateError("profile before init");
+    events.clear();
+    profiling = true;
+    try {
+        for (int q = 0; q < nsteps; ++q) {
+            if (fp32) stepOnce<float>();
+            else stepOnce<double>();
+        }
+    } catch (...) {
+        profiling = false;
+        throw;
+    }
+    profiling = false;
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    double sum[4] = {0, 0, 0, 0};
+    int cnt[4] = {0, 0, 0, 0};
+    for (auto& e : events) {
+        float t = 0;
+        hipCheck(hipEventElapsedTime(&t, e.a, e.b), "elapsed");
+        sum[e.group] += t;
+        cnt[e.group] += 1;
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    events.clear();
+    for (int g = 0; g < 4; ++g) {
+        ms[g] = cnt[g] ? sum[g] / cnt[g] : 0.0;
+        launches[g] = nsteps ? cnt[g] / nsteps : 0;
+    }
+    readScalars();
+}
+
+void Case::init() {
+    setupBoundaries();
+    categorizeNodes();
+    if (fp32) { allocFields<float>(); std::vector<float> h; hostInitialField(h); uploadState(h); }
+    else { allocFields<double>(); std::vector<double> h; hostInitialField(h); uploadState(h); }
+    readScalars();
+    initialized = true;
+}
+
+void Case::setInitialField(const double* U5) {
+    auto fill = [&](auto& h) {
+        using T = typename std::decay_t<decltype(h)>::value_type;
+        h.assign(5 * L.fstride, (T)0);
+        size_t N = L.nodes();
+        for (int q = 0; q < 5; ++q)
+            for (int k = 0; k < L.NK; ++k)
+                for (int j = 0; j < L.NJ; ++j)
+                    for (int i = 0; i < L.NI; ++i)
+                        h[L.off(i, j, k) + q * L.fstride] = (T)U5[q * N + ((size_t)k * L.NJ + j) * L.NI + i];
+    };
+    if (fp32) { std::vector<float> h; fill(h); uploadState(h); }
+    else { std::vector<double> h; fill(h); uploadState(h); }
+    readScalars();
+}
+
+void Case::readScalars() {
+    StepScalars s;
+    hipCheck(hipMemcpyAsync(&s, scal, sizeof(s), hipMemcpyDeviceToHost, stream), "copy scalars");
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    t = s.t;
+    dt = s.dt;
+    level = s.level;
+    nonFinite = s.nonFinite != 0;
+    // drain the norm-history ring
+    uint64_t have = normsHost[0].size();
+    if (s.histCount > have) {
+        std::vector<double> h((size_t)histCap * 5);
+        hipCheck(hipMemcpy(h.data(), hist, h.size() * sizeof(double), hipMemcpyDeviceToHost), "copy hist");
+        for (uint64_t q = have; q < s.histCount; ++q) {
+            int slot = (int)(q % (uint64_t)histCap);
+            for (int v = 0; v < 5; ++v) normsHost[v].push_back(h[5 * slot + v]);
+        }
+    }
+}
+
+void Case::step(int nsteps) {
+    if (!initialized) throw StateError("step before init");
+    if (!filterDisabled()) throw Unsupported("the periodic density filter (filterIntervals) is not implemented on the device");
+    int done = 0;
+    while (done < nsteps) {
+        int batch = std::min(nsteps - done, histCap);
+        for (int q = 0; q < batch; ++q) {
+            if (fp32) stepOnce<float>();
+            else stepOnce<double>();
+        }
+        hipCheck(hipGetLastError(), "step kernels");
+        readScalars();
+        done += batch;
+    }
+}
+
+bool Case::filterDisabled() const {
+    for (int q = 0; q < cfg.n_filter_intervals; ++q)
+        if (cfg.filter_intervals[q] > 0) return false;
+    return true;
+}
+
+void Case::getField(int which, double* out) {
+    if (!initialized) throw StateError("get_field before init");
+    size_t N = L.nodes();
+    auto copyOut = [&](const void* base, int q) {
+        if (fp32) {
+            std::vector<float> h(L.fstride);
+            hipCheck(hipMemcpy(h.data(), (const float*)base + (size_t)q * L.fstride, L.fstride * sizeof(float),
+                               hipMemcpyDeviceToHost), "copy field");
+            for (int k = 0; k < L.NK; ++k)
+                for (int j = 0; j < L.NJ; ++j)
+                    for (int i = 0; i < L.NI; ++i) out[((size_t)k * L.NJ + j) * L.NI + i] = h[L.off(i, j, k)];
+        } else {
+            std::vector<double> h(L.fstride);
+            hipCheck(hipMemcpy(h.data(), (const double*)base + (size_t)q * L.fstride, L.fstride * sizeof(double),
+                               hipMemcpyDeviceToHost), "copy field");
+            for (int k = 0; k < L.NK; ++k)
+                for (int j = 0; j < L.NJ; ++j)
+                    for (int i = 0; i < L.NI; ++i) out[((size_t)k * L.NJ + j) * L.NI + i] = h[L.off(i, j, k)];
+        }
+        (void)N;
+    };
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    if (which < 5) { copyOut(U, which); return; }
+    // primitive and transport values as the reference keeps them: derived from the state, pre-filter
+    // values at filter nodes, initial values at untouched solid nodes
+    if (fp32) {
+        launchDerive<float>(stream, L, P, (const float*)U, (float*)prim, cls, W, (const float*)rhoPre, (const float*)rhoEPre);
+        if (nInactive > 0)
+            hipLaunchKernelGGL(k_patch_inactive<float>, dim3(nblk(nInactive, 128)), dim3(128), 0, stream, L, P, (float*)prim,
+                               inactiveList, inactiveIC, nInactive);
+    } else {
+        launchDerive<double>(stream, L, P, (const double*)U, (double*)prim, cls, W, (const double*)rhoPre,
+                             (const double*)rhoEPre);
+        if (nInactive > 0)
+            hipLaunchKernelGGL(k_patch_inactive<double>, dim3(nblk(nInactive, 128)), dim3(128), 0, stream, L, P,
+                               (double*)prim, inactiveList, inactiveIC, nInactive);
+    }
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    copyOut(prim, which - 5);
+}
+
+void Case::nodeTypes(uint8_t* out) {
+    if (!initialized) throw StateError("node types before init");
+    std::vector<uint8_t> h((size_t)L.NK * L.plane);
+    hipCheck(hipMemcpy(h.data(), cls, h.size(), hipMemcpyDeviceToHost), "copy cls");
+    for (int k = 0; k < L.NK; ++k)
+        for (int j = 0; j < L.NJ; ++j)
+            for (int i = 0; i < L.NI; ++i) out[((size_t)k * L.NJ + j) * L.NI + i] = h[L.off(i, j, k)] & CLS_TYPE_MASK;
+}
+
+static std::vector<int64_t> sortedByReference(const std::vector<int64_t>& nodes, int NI, int NJ, int NK,
+                                              std::vector<size_t>& order) {
+    // reference index (i*NJ + j)*NK + k
+    std::vector<int64_t> key(nodes.size());
+    for (size_t q = 0; q < nodes.size(); ++q) {
+        int64_t n = nodes[q];
+        int64_t i = n % NI, r = n / NI, j = r % NJ, k = r / NJ;
+        key[q] = (i * NJ + j) * NK + k;
+    }
+    order.resize(nodes.size());
+    for (size_t q = 0; q < order.size(); ++q) order[q] = q;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return key[a] < key[b]; });
+    return key;
+}
+
+void Case::ghosts(int32_t* ijk, double* bi, double* ip, int32_t* box) {
+    if (!initialized) throw StateError("ghosts before init");
+    std::vector<int64_t> nodes(nGhost);
+    std::vector<double> hb(3 * nGhost), hp(3 * nGhost);
+    std::vector<int32_t> hbox(3 * nGhost);
+    if (nGhost) {
+        hipCheck(hipMemcpy(nodes.data(), ghostList, nGhost * sizeof(int64_t), hipMemcpyDeviceToHost), "copy ghosts");
+        hipCheck(hipMemcpy(hb.data(), G.bi, hb.size() * sizeof(double), hipMemcpyDeviceToHost), "copy bi");
+        hipCheck(hipMemcpy(hp.data(), G.ip, hp.size() * sizeof(double), hipMemcpyDeviceToHost), "copy ip");
+        hipCheck(hipMemcpy(hbox.data(), G.boxLo, hbox.size() * sizeof(int32_t), hipMemcpyDeviceToHost), "copy box");
+    }
+    std::vector<size_t> order;
+    sortedByReference(nodes, L.NI, L.NJ, L.NK, order);
+    for (size_t q = 0; q < order.size(); ++q) {
+        size_t g = order[q];
+        int64_t n = nodes[g];
+        if (ijk) { ijk[3 * q] = (int32_t)(n % L.NI); ijk[3 * q + 1] = (int32_t)((n / L.NI) % L.NJ); ijk[3 * q + 2] = (int32_t)(n / L.NI / L.NJ); }
+        for (int c = 0; c < 3; ++c) {
+            if (bi) bi[3 * q + c] = hb[3 * g + c];
+            if (ip) ip[3 * q + c] = hp[3 * g + c];
+            if (box) box[3 * q + c] = hbox[3 * g + c];
+        }
+    }
+}
+
+void Case::filterNodes(int32_t* ijk) {
+    if (!initialized) throw StateError("filter nodes before init");
+    std::vector<int64_t> nodes(nFilter);
+    if (nFilter) hipCheck(hipMemcpy(nodes.data(), filterList, nFilter * sizeof(int64_t), hipMemcpyDeviceToHost), "copy filter");
+    std::vector<size_t> order;
+    sortedByReference(nodes, L.NI, L.NJ, L.NK, order);
+    for (size_t q = 0; q < order.size(); ++q) {
+        int64_t n = nodes[order[q]];
+        ijk[3 * q] = (int32_t)(n % L.NI);
+        ijk[3 * q + 1] = (int32_t)((n / L.NI) % L.NJ);
+        ijk[3 * q + 2] = (int32_t)(n / L.NI / L.NJ);
+    }
+}
+
+int64_t Case::interiorCount() {
+    std::vector<uint8_t> h((size_t)L.NK * L.plane);
+    hipCheck(hipMemcpy(h.data(), cls, h.size(), hipMemcpyDeviceToHost), "copy cls");
+    int64_t n = 0;
+    for (int k = 0; k < L.NK; ++k)
+        for (int j = 0; j < L.NJ; ++j)
+            for (int i = 0; i < L.NI; ++i) n += (h[L.off(i, j, k)] & CLS_TYPE_MASK) == NS_FLUID_INTERIOR;
+    return n;
+}
+
+void Case::totals(double out[5]) {
+    int nb = 256;
+    double* part = partials;  // reuse (1024 x 10 doubles >= 256 x 5)
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    if (fp32) hipLaunchKernelGGL(k_totals<float>, dim3(nb), dim3(256), 0, stream, L, (const float*)U, cls, part);
+    else hipLaunchKernelGGL(k_totals<double>, dim3(nb), dim3(256), 0, stream, L, (const double*)U, cls, part);
+    std::vector<double> h((size_t)nb * 5);
+    hipCheck(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream), "copy totals");
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    double vol = P.hx * P.hy * P.hz;
+    for (int q = 0; q < 5; ++q) {
+        double s = 0;
+        for (int b = 0; b < nb; ++b) s += h[(size_t)b * 5 + q];
+        out[q] = s * vol;
+    }
+}
+
+}  // namespace nsfsi

@@ -1,0 +1,44 @@
+/*
+ * stage.h — launch wrappers of the RK4 stage kernels and of the end-of-step reductions.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ns_internal.h"
+
+namespace nsfsi {
+
+// Device-resident step scalars: read by every stage kernel, written only by the finalize kernel, so a
+// sequence of steps runs without host synchronisation.
+struct StepScalars {
+    double t;
+    double dt;
+    uint64_t level;
+    uint64_t histCount;
+    int32_t nonFinite;
+    int32_t pad;
+};
+
+// ---- unfused stage (stage_plain.hip)
+template <class T>
+void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T* prim, const uint8_t* cls, const Window& W,
+                  const T* rhoPre, const T* rhoEPre);
+template <class T>
+void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
+                    T* Sout, T* A, const uint8_t* cls, const double* dtdev);
+
+// ---- end-of-step reductions (reduce.hip)
+template <class T>
+void launchStepReduce(hipStream_t s, const Layout& L, const Params& P, const T* U, const T* A, const uint8_t* cls,
+                      const Window& W, const T* rhoPre, const T* rhoEPre, int withNorms, double* partials, int nBlocks);
+template <class T>
+void launchInactiveReduce(hipStream_t s, const Layout& L, const Params& P, const T* A, const int64_t* nodes,
+                          const double* icPrim, int64_t n, double* partial);
+void launchFinalize(hipStream_t s, const Params& P, const double* partials, int nPartials, int mode, int convHistory,
+                    int stopEndTime, double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap);
+int reducePartialSlots();
+
+}  // namespace nsfsi

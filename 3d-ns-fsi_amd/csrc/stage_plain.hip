@@ -1,0 +1,218 @@
+/*
+ * stage_plain.hip — the unfused RK4 stage: a primitive-variable pass (7 stored fields) followed by a
+ * per-node 19-point right-hand side that follows the reference's flux expressions term by term
+ * (compute_RK4_step_*, src/Solver.cpp:181-418) and the stage combine.
+ *
+ * This is the straightforward "plain" device path: one thread per node, neighbours through the cache
+ * hierarchy. The fused LDS-tiled stage (stage_fused.hip) is checked against it.
+ *
+ * Stage combine (3-register form, src/Solver.cpp:422-507): with U the state at t, A the accumulator and
+ * S the stage state,
+ *   stage 1: S1 = U + dt/2 k1,  A = U + dt/6 k1
+ *   stage 2: S2 = U + dt/2 k2,  A += dt/3 k2
+ *   stage 3: S3 = U + dt k3,    A += dt/3 k3
+ *   stage 4: A += dt/6 k4       (A becomes the state at t+dt)
+ * Only FluidInterior nodes are combined; every other node of the output copies the buffer that the
+ * reference's "current" array holds at that point (stage 1: A, which still holds the state of the
+ * previous step; later stages: the stage input), so boundary kernels see the same stale values the
+ * reference sees (ghost values two boundary applications old at stage 1).
+ */
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "stage.h"
+
+namespace nsfsi {
+
+namespace {
+
+template <class T>
+__global__ void k_derive(Layout L, Params P, const T* S, T* prim, const uint8_t* cls, Window W, const T* rhoPre,
+                         const T* rhoEPre) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.NI) return;
+    size_t o = L.off(i, j, k);
+    Eos<T> e = Eos<T>::make(P);
+    T rho = S[o], rE = S[o + 4 * L.fstride];
+    if (cls[o] & CLS_FILTER) {
+        size_t w = W.idx(i, j, k);
+        rho = rhoPre[w];
+        rE = rhoEPre[w];
+    }
+    T u, v, w, p, Tt, mu;
+    derivePrims(e, rho, S[o + L.fstride], S[o + 2 * L.fstride], S[o + 3 * L.fstride], rE, u, v, w, p, Tt, mu);
+    prim[o] = u;
+    prim[o + L.fstride] = v;
+    prim[o + 2 * L.fstride] = w;
+    prim[o + 3 * L.fstride] = p;
+    prim[o + 4 * L.fstride] = Tt;
+    prim[o + 5 * L.fstride] = mu;
+    prim[o + 6 * L.fstride] = mu * e.pf;
+}
+
+template <class T, int STAGE>
+__global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, const T* U, T* Sout, T* A, const uint8_t* cls,
+                            const double* dtdev) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.NI) return;
+    const size_t o = L.off(i, j, k);
+    const size_t F = L.fstride;
+    T* out = STAGE == 4 ? A : Sout;
+    if ((cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) {
+        const T* src = STAGE == 1 ? A : Sin;
+        for (int q = 0; q < 5; ++q) out[o + q * F] = src[o + q * F];
+        return;
+    }
+    const T dx = (T)P.hx, dy = (T)P.hy, dz = (T)P.hz;
+    const T n2dx = (T)-1 / ((T)2 * dx), n2dy = (T)-1 / ((T)2 * dy), n2dz = (T)-1 / ((T)2 * dz);
+    const T p23xx = (T)2 / ((T)3 * dx * dx), p23yy = (T)2 / ((T)3 * dy * dy), p23zz = (T)2 / ((T)3 * dz * dz);
+    const T p12xx = (T)1 / ((T)2 * dx * dx), p12yy = (T)1 / ((T)2 * dy * dy), p12zz = (T)1 / ((T)2 * dz * dz);
+    const T c6xy = (T)-1 / ((T)6 * dx * dy), c4xy = (T)1 / ((T)4 * dx * dy);
+    const T c6xz = (T)-1 / ((T)6 * dx * dz), c4xz = (T)1 / ((T)4 * dx * dz);
+    const T c6yz = (T)-1 / ((T)6 * dy * dz), c4yz = (T)1 / ((T)4 * dy * dz);
+    const T H0 = (T)(1 / (P.gamma - 1));
+    const T* ru = Sin + F;
+    const T* rv = Sin + 2 * F;
+    const T* rw = Sin + 3 * F;
+    const T* E = Sin + 4 * F;
+    const T* u = prim;
+    const T* v = prim + F;
+    const T* w = prim + 2 * F;
+    const T* p = prim + 3 * F;
+    const T* Tt = prim + 4 * F;
+    const T* mu = prim + 5 * F;
+    const T* ka = prim + 6 * F;
+    const long X = 1, Y = L.pitch, Z = (long)L.plane;
+    const size_t e_ = o + X, w_ = o - X, n_ = o + Y, s_ = o - Y, u_ = o + Z, d_ = o - Z;
+    T kq[5];
+    // continuity (src/Solver.cpp:191-194)
+    kq[0] = -(ru[e_] - ru[w_]) / ((T)2 * dx) + -(rv[n_] - rv[s_]) / ((T)2 * dy) + -(rw[u_] - rw[d_]) / ((T)2 * dz);
+    // momentum (src/Solver.cpp:225-241, 272-288, 320-337)
+    {
+        T cX = n2dx * (ru[e_] * u[e_] + p[e_] - ru[w_] * u[w_] - p[w_]);
+        T cY = n2dy * (ru[n_] * v[n_] - ru[s_] * v[s_]);
+        T cZ = n2dz * (ru[u_] * w[u_] - ru[d_] * w[d_]);
+        T vXX = p23xx * ((mu[e_] + mu[o]) * (u[e_] - u[o]) - (mu[w_] + mu[o]) * (u[o] - u[w_]));
+        T vYY = p12yy * ((mu[n_] + mu[o]) * (u[n_] - u[o]) - (mu[s_] + mu[o]) * (u[o] - u[s_]));
+        T vZZ = p12zz * ((mu[u_] + mu[o]) * (u[u_] - u[o]) - (mu[d_] + mu[o]) * (u[o] - u[d_]));
+        T vXY = c4xy * (mu[n_] * (v[n_ + X] - v[n_ - X]) - mu[s_] * (v[s_ + X] - v[s_ - X]));
+        T vYX = c6xy * (mu[e_] * (v[e_ + Y] - v[e_ - Y]) - mu[w_] * (v[w_ + Y] - v[w_ - Y]));
+        T vXZ = c4xz * (mu[u_] * (w[u_ + X] - w[u_ - X]) - mu[d_] * (w[d_ + X] - w[d_ - X]));
+        T vZX = c6xz * (mu[e_] * (w[e_ + Z] - w[e_ - Z]) - mu[w_] * (w[w_ + Z] - w[w_ - Z]));
+        kq[1] = cX + cY + cZ + vXX + vYY + vZZ + vXY + vYX + vXZ + vZX;
+    }
+    {
+        T cX = n2dx * (rv[e_] * u[e_] - rv[w_] * u[w_]);
+        T cY = n2dy * (rv[n_] * v[n_] + p[n_] - rv[s_] * v[s_] - p[s_]);
+        T cZ = n2dz * (rv[u_] * w[u_] - rv[d_] * w[d_]);
+        T vXX = p12xx * ((mu[e_] + mu[o]) * (v[e_] - v[o]) - (mu[w_] + mu[o]) * (v[o] - v[w_]));
+        T vYY = p23yy * ((mu[n_] + mu[o]) * (v[n_] - v[o]) - (mu[s_] + mu[o]) * (v[o] - v[s_]));
+        T vZZ = p12zz * ((mu[u_] + mu[o]) * (v[u_] - v[o]) - (mu[d_] + mu[o]) * (v[o] - v[d_]));
+        T vYX = c4xy * (mu[e_] * (u[e_ + Y] - u[e_ - Y]) - mu[w_] * (u[w_ + Y] - u[w_ - Y]));
+        T vXY = c6xy * (mu[n_] * (u[n_ + X] - u[n_ - X]) - mu[s_] * (u[s_ + X] - u[s_ - X]));
+        T vYZ = c4yz * (mu[u_] * (w[u_ + Y] - w[u_ - Y]) - mu[d_] * (w[d_ + Y] - w[d_ - Y]));
+        T vZY = c6yz * (mu[n_] * (w[n_ + Z] - w[n_ - Z]) - mu[s_] * (w[s_ + Z] - w[s_ - Z]));
+        kq[2] = cX + cY + cZ + vXX + vYY + vZZ + vYX + vXY + vYZ + vZY;
+    }
+    {
+        T cX = n2dx * (rw[e_] * u[e_] - rw[w_] * u[w_]);
+        T cY = n2dy * (rw[n_] * v[n_] - rw[s_] * v[s_]);
+        T cZ = n2dz * (rw[u_] * w[u_] + p[u_] - rw[d_] * w[d_] - p[d_]);
+        T vXX = p12xx * ((mu[e_] + mu[o]) * (w[e_] - w[o]) - (mu[w_] + mu[o]) * (w[o] - w[w_]));
+        T vYY = p12yy * ((mu[n_] + mu[o]) * (w[n_] - w[o]) - (mu[s_] + mu[o]) * (w[o] - w[s_]));
+        T vZZ = p23zz * ((mu[u_] + mu[o]) * (w[u_] - w[o]) - (mu[d_] + mu[o]) * (w[o] - w[d_]));
+        T vZX = c4xz * (mu[e_] * (u[e_ + Z] - u[e_ - Z]) - mu[w_] * (u[w_ + Z] - u[w_ - Z]));
+        T vXZ = c6xz * (mu[u_] * (u[u_ + X] - u[u_ - X]) - mu[d_] * (u[d_ + X] - u[d_ - X]));
+        T vZY = c4yz * (mu[n_] * (v[n_ + Z] - v[n_ - Z]) - mu[s_] * (v[s_ + Z] - v[s_ - Z]));
+        T vYZ = c6yz * (mu[u_] * (v[u_ + Y] - v[u_ - Y]) - mu[d_] * (v[d_ + Y] - v[d_ - Y]));
+        kq[3] = cX + cY + cZ + vXX + vYY + vZZ + vZX + vXZ + vZY + vYZ;
+    }
+    // energy (src/Solver.cpp:378-416)
+    {
+        const T mP = mu[o], mW = mu[w_], mE = mu[e_], mS = mu[s_], mN = mu[n_], mD = mu[d_], mU = mu[u_];
+        const T uP = u[o], uW = u[w_], uE = u[e_], uS = u[s_], uN = u[n_], uD = u[d_], uU = u[u_];
+        const T vP = v[o], vW = v[w_], vE = v[e_], vS = v[s_], vN = v[n_], vD = v[d_], vU = v[u_];
+        const T wP = w[o], wW = w[w_], wE = w[e_], wS = w[s_], wN = w[n_], wD = w[d_], wU = w[u_];
+        const T kP = ka[o], TP = Tt[o];
+        T inv = n2dx * ((H0 + E[e_] + p[e_]) * uE - (H0 + E[w_] + p[w_]) * uW)
+              + n2dy * ((H0 + E[n_] + p[n_]) * vN - (H0 + E[s_] + p[s_]) * vS)
+              + n2dz * ((H0 + E[u_] + p[u_]) * wU - (H0 + E[d_] + p[d_]) * wD);
+        T vx = p23xx * ((mE * uE + mP * uP) * (uE - uP) - (mW * uW + mP * uP) * (uP - uW))
+             + p12xx * ((mE * vE + mP * vP) * (vE - vP) - (mW * vW + mP * vP) * (vP - vW))
+             + p12xx * ((mE * wE + mP * wP) * (wE - wP) - (mW * wW + mP * wP) * (wP - wW))
+             + c6xy * (mE * uE * (v[e_ + Y] - v[e_ - Y]) - mW * uW * (v[w_ + Y] - v[w_ - Y]))
+             + c6xz * (mE * uE * (w[e_ + Z] - w[e_ - Z]) - mW * uW * (w[w_ + Z] - w[w_ - Z]))
+             + c4xy * (mE * vE * (u[e_ + Y] - u[e_ - Y]) - mW * vW * (u[w_ + Y] - u[w_ - Y]))
+             + c4xz * (mE * wE * (u[e_ + Z] - u[e_ - Z]) - mW * wW * (u[w_ + Z] - u[w_ - Z]));
+        T vy = p12yy * ((mN * uN + mP * uP) * (uN - uP) - (mS * uS + mP * uP) * (uP - uS))
+             + p23yy * ((mN * vN + mP * vP) * (vN - vP) - (mS * vS + mP * vP) * (vP - vS))
+             + p12yy * ((mN * wN + mP * wP) * (wN - wP) - (mS * wS + mP * wP) * (wP - wS))
+             + c6xy * (mN * vN * (u[n_ + X] - u[n_ - X]) - mS * vS * (u[s_ + X] - u[s_ - X]))
+             + c6yz * (mN * vN * (w[n_ + Z] - w[n_ - Z]) - mS * vS * (w[s_ + Z] - w[s_ - Z]))
+             + c4xy * (mN * uN * (v[n_ + X] - v[n_ - X]) - mS * uS * (v[s_ + X] - v[s_ - X]))
+             + c4yz * (mN * wN * (v[n_ + Z] - v[n_ - Z]) - mS * wS * (v[s_ + Z] - v[s_ - Z]));
+        T vz = p12zz * ((mU * uU + mP * uP) * (uU - uP) - (mD * uD + mP * uP) * (uP - uD))
+             + p12zz * ((mU * vU + mP * vP) * (vU - vP) - (mD * vD + mP * vP) * (vP - vD))
+             + p23zz * ((mU * wU + mP * wP) * (wU - wP) - (mD * wD + mP * wP) * (wP - wD))
+             + c6xz * (mU * wU * (u[u_ + X] - u[u_ - X]) - mD * wD * (u[d_ + X] - u[d_ - X]))
+             + c6yz * (mU * wU * (v[u_ + Y] - v[u_ - Y]) - mD * wD * (v[d_ + Y] - v[d_ - Y]))
+             + c4xz * (mU * uU * (w[u_ + X] - w[u_ - X]) - mD * uD * (w[d_ + X] - w[d_ - X]))
+             + c4yz * (mU * vU * (w[u_ + Y] - w[u_ - Y]) - mD * vD * (w[d_ + Y] - w[d_ - Y]));
+        T heat = p12xx * ((ka[e_] + kP) * (Tt[e_] - TP) - (ka[w_] + kP) * (TP - Tt[w_]))
+               + p12yy * ((ka[n_] + kP) * (Tt[n_] - TP) - (ka[s_] + kP) * (TP - Tt[s_]))
+               + p12zz * ((ka[u_] + kP) * (Tt[u_] - TP) - (ka[d_] + kP) * (TP - Tt[d_]));
+        kq[4] = inv + vx + vy + vz + heat;
+    }
+    const double dtd = *dtdev;
+    if (STAGE == 1) {
+        const T h2 = (T)(dtd / 2), h6 = (T)(dtd / 6);
+        for (int q = 0; q < 5; ++q) {
+            T u0 = U[o + q * F];
+            Sout[o + q * F] = u0 + h2 * kq[q];
+            A[o + q * F] = u0 + h6 * kq[q];
+        }
+    } else if (STAGE == 2 || STAGE == 3) {
+        const T hs = (T)(STAGE == 2 ? dtd / 2 : dtd), h3 = (T)(dtd / 3);
+        for (int q = 0; q < 5; ++q) {
+            Sout[o + q * F] = U[o + q * F] + hs * kq[q];
+            A[o + q * F] = A[o + q * F] + h3 * kq[q];
+        }
+    } else {
+        const T h6 = (T)(dtd / 6);
+        for (int q = 0; q < 5; ++q) A[o + q * F] = A[o + q * F] + h6 * kq[q];
+    }
+}
+
+}  // namespace
+
+template <class T>
+void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T* prim, const uint8_t* cls, const Window& W,
+                  const T* rhoPre, const T* rhoEPre) {
+    dim3 blk(64), grd((L.NI + 63) / 64, L.NJ, L.NK);
+    hipLaunchKernelGGL(k_derive<T>, grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
+}
+
+template <class T>
+void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
+                    T* Sout, T* A, const uint8_t* cls, const double* dtdev) {
+    dim3 blk(64), grd((L.NI + 63) / 64, L.NJ, L.NK);
+    switch (stage) {
+        case 1: hipLaunchKernelGGL((k_rhs_plain<T, 1>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev); break;
+        case 2: hipLaunchKernelGGL((k_rhs_plain<T, 2>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev); break;
+        case 3: hipLaunchKernelGGL((k_rhs_plain<T, 3>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev); break;
+        default: hipLaunchKernelGGL((k_rhs_plain<T, 4>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev); break;
+    }
+}
+
+template void launchDerive<double>(hipStream_t, const Layout&, const Params&, const double*, double*, const uint8_t*,
+                                   const Window&, const double*, const double*);
+template void launchDerive<float>(hipStream_t, const Layout&, const Params&, const float*, float*, const uint8_t*,
+                                  const Window&, const float*, const float*);
+template void launchRhsPlain<double>(hipStream_t, int, const Layout&, const Params&, const double*, const double*,
+                                     const double*, double*, double*, const uint8_t*, const double*);
+template void launchRhsPlain<float>(hipStream_t, int, const Layout&, const Params&, const float*, const float*,
+                                    const float*, float*, float*, const uint8_t*, const double*);
+
+}  // namespace nsfsi

@@ -1,0 +1,360 @@
+"""Python host layer over libnsfsi.so (the C-ABI in include/ns_fsi.h).
+
+Mirrors the reference's host surface (Frederik-Kris/3D-NS-FSI): ``ConfigSettings`` (src/ConfigSettings.h),
+``Solver`` with ``initialize`` / ``marchTimeStep`` / ``dt`` / ``getConvergenceHistory`` and a ``mesh``
+view (src/Solver.h:17-31, src/Mesh.h:79-89), and ``Simulation.run`` (src/Simulation.h:17-32).
+
+The device library is the only compute path: if libnsfsi.so is missing, or no gfx950 GPU is present,
+the calls raise; nothing here falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnsfsi.so")
+
+NS_MAX_FILTER = 16
+
+STATUS = {
+    0: "NS_OK", 1: "NS_ERR_CONFIG", 2: "NS_ERR_GEOMETRY", 3: "NS_ERR_NONFINITE", 4: "NS_ERR_HIP",
+    5: "NS_ERR_ARGUMENT", 6: "NS_ERR_STATE", 7: "NS_ERR_UNSUPPORTED",
+}
+BC = {"inlet": 0, "outlet": 1, "periodic": 2, "symmetry": 3}
+BODY = {"none": 0, "cylinder": 1, "sphere": 2}
+IC = {"stagnation": 0, "uniform_oblique": 1, "freestream_gaussian": 2, "freestream_uniform": 3}
+FIELDS = ["rho", "rho_u", "rho_v", "rho_w", "rho_E", "u", "v", "w", "p", "T", "mu", "kappa"]
+NODE_TYPES = ["FluidInterior", "FluidEdge", "FluidGhost", "SolidInactive", "SolidGhost"]
+
+
+class NSConfig(C.Structure):
+    """ctypes image of ``ns_config`` (include/ns_fsi.h)."""
+
+    _fields_ = [
+        ("NI", C.c_int32), ("NJ", C.c_int32), ("NK", C.c_int32),
+        ("stop_criterion", C.c_int32),
+        ("stop_time_level", C.c_uint64),
+        ("t_end", C.c_double),
+        ("conv_stability_limit", C.c_double),
+        ("visc_stability_limit", C.c_double),
+        ("n_filter_intervals", C.c_int32),
+        ("filter_intervals", C.c_uint32 * NS_MAX_FILTER),
+        ("n_filter_change_times", C.c_int32),
+        ("filter_change_times", C.c_double * NS_MAX_FILTER),
+        ("status_report_interval", C.c_double),
+        ("save_density", C.c_int32), ("save_momentum", C.c_int32), ("save_energy", C.c_int32),
+        ("save_velocity", C.c_int32), ("save_pressure", C.c_int32), ("save_temperature", C.c_int32),
+        ("save_viscosity", C.c_int32), ("save_thermal_cond", C.c_int32),
+        ("save_ic", C.c_int32), ("save_final", C.c_int32), ("save_intervals", C.c_int32),
+        ("save_period", C.c_double), ("save_intervals_start_time", C.c_double), ("save_intervals_end_time", C.c_double),
+        ("conv_history", C.c_int32),
+        ("gamma", C.c_double), ("Pr", C.c_double), ("R", C.c_double), ("Re", C.c_double),
+        ("sutherlands_C2", C.c_double), ("M_0", C.c_double), ("T_0", C.c_double),
+        ("L_x", C.c_double), ("L_y", C.c_double), ("L_z", C.c_double),
+        ("machine_precision_buffer", C.c_double),
+        ("bc", C.c_int32 * 6),
+        ("body_type", C.c_int32),
+        ("body_axis", C.c_int32),
+        ("body_centre", C.c_double * 3),
+        ("body_radius", C.c_double),
+        ("ic_type", C.c_int32),
+        ("ic_seed", C.c_uint64),
+        ("ic_amplitude", C.c_double),
+        ("inlet_ramp_time", C.c_double),
+        ("precision", C.c_int32),
+        ("kahan", C.c_int32),
+    ]
+
+
+class NSError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Loads libnsfsi.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    H = C.c_void_p
+    L.ns_config_default.argtypes = [C.POINTER(NSConfig)]
+    L.ns_config_default.restype = None
+    L.ns_config_parse.argtypes = [C.c_char_p, C.POINTER(NSConfig), C.c_char_p, C.c_size_t]
+    L.ns_create.argtypes = [C.POINTER(NSConfig), C.POINTER(H)]
+    L.ns_init.argtypes = [H]
+    L.ns_set_initial_field.argtypes = [H, C.POINTER(C.c_double)]
+    L.ns_step.argtypes = [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+    L.ns_run.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    L.ns_get_time.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+    L.ns_get_field.argtypes = [H, C.c_int32, C.POINTER(C.c_double)]
+    L.ns_get_node_types.argtypes = [H, C.POINTER(C.c_uint8)]
+    L.ns_get_counts.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.ns_get_ghosts.argtypes = [H, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                C.POINTER(C.c_int32)]
+    L.ns_get_filter_nodes.argtypes = [H, C.POINTER(C.c_int32)]
+    L.ns_norm_history.argtypes = [H, C.c_int32, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]
+    L.ns_totals.argtypes = [H, C.POINTER(C.c_double)]
+    L.ns_get_geometry.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.ns_profile.argtypes = [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    L.ns_last_error.argtypes = [H]
+    L.ns_last_error.restype = C.c_char_p
+    L.ns_destroy.argtypes = [H]
+    L.ns_destroy.restype = None
+    L.ns_abi_version.restype = C.c_int32
+    for name in ("ns_config_parse", "ns_create", "ns_init", "ns_set_initial_field", "ns_step", "ns_run", "ns_get_time",
+                 "ns_get_field", "ns_get_node_types", "ns_get_counts", "ns_get_ghosts", "ns_get_filter_nodes",
+                 "ns_norm_history", "ns_totals", "ns_get_geometry", "ns_profile"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def default_config() -> NSConfig:
+    c = NSConfig()
+    lib().ns_config_default(C.byref(c))
+    return c
+
+
+def make_config(**kw) -> NSConfig:
+    """Default configuration (the reference's shipped ConfigFile) with keyword overrides.
+
+    Names follow ns_config; ``bc`` takes 6 names (x-min .. z-max), ``body_type``/``ic_type`` take names,
+    ``filter_intervals``/``filter_change_times`` take lists, ``body_centre`` a 3-sequence,
+    ``precision`` "fp64"/"fp32".
+    """
+    c = default_config()
+    for k, v in kw.items():
+        if k == "bc":
+            for q, name in enumerate(v):
+                c.bc[q] = BC[name] if isinstance(name, str) else int(name)
+        elif k == "body_type":
+            c.body_type = BODY[v] if isinstance(v, str) else int(v)
+        elif k == "body_axis":
+            c.body_axis = "xyz".index(v) if isinstance(v, str) else int(v)
+        elif k == "ic_type":
+            c.ic_type = IC[v] if isinstance(v, str) else int(v)
+        elif k == "body_centre":
+            for q in range(3):
+                c.body_centre[q] = float(v[q])
+        elif k == "filter_intervals":
+            c.n_filter_intervals = len(v)
+            for q, x in enumerate(v):
+                c.filter_intervals[q] = int(x)
+        elif k == "filter_change_times":
+            c.n_filter_change_times = len(v)
+            for q, x in enumerate(v):
+                c.filter_change_times[q] = float(x)
+        elif k == "precision":
+            c.precision = {"fp64": 0, "fp32": 1}[v] if isinstance(v, str) else int(v)
+        elif k == "stop_criterion":
+            c.stop_criterion = {"timesteps": 0, "end_time": 1}[v] if isinstance(v, str) else int(v)
+        elif k == "conv_history":
+            c.conv_history = {"none": 0, "density": 1, "all": 2}[v] if isinstance(v, str) else int(v)
+        else:
+            if not hasattr(c, k):
+                raise KeyError(k)
+            setattr(c, k, v)
+    c.machine_precision_buffer = np.finfo(np.float64).eps * (c.L_x + c.L_y + c.L_z)
+    return c
+
+
+def parse_config(text: str) -> NSConfig:
+    """ConfigSettings::loadSettings on a ConfigFile text (src/ConfigSettings.cpp:18-28)."""
+    c = NSConfig()
+    err = C.create_string_buffer(512)
+    s = lib().ns_config_parse(text.encode(), C.byref(c), err, 512)
+    if s != 0:
+        raise NSError(s, err.value.decode())
+    return c
+
+
+class ConfigSettings:
+    """ConfigSettings(filename) of the reference: parses a ConfigFile; ``errorOccurred`` instead of raising."""
+
+    def __init__(self, filename: str):
+        self.errorOccurred = False
+        self.message = ""
+        try:
+            with open(filename) as f:
+                self.config = parse_config(f.read())
+        except (OSError, NSError) as e:
+            self.errorOccurred = True
+            self.message = str(e)
+            self.config = None
+
+
+class Mesh:
+    """Read-only view of the device mesh (Mesh public members, src/Mesh.h:79-89)."""
+
+    def __init__(self, solver: "Solver"):
+        self._s = solver
+        c = solver.config
+        self.NI, self.NJ, self.NK = c.NI, c.NJ, c.NK
+        self.nNodesTotal = self.NI * self.NJ * self.NK
+
+    def _geometry(self):
+        sp = np.zeros(3)
+        of = np.zeros(3)
+        self._s._check(lib().ns_get_geometry(self._s._h, _dptr(sp), _dptr(of)))
+        return sp, of
+
+    @property
+    def dx(self):
+        return self._geometry()[0][0]
+
+    @property
+    def dy(self):
+        return self._geometry()[0][1]
+
+    @property
+    def dz(self):
+        return self._geometry()[0][2]
+
+    @property
+    def positionOffset(self):
+        return self._geometry()[1]
+
+    def field(self, name: str) -> np.ndarray:
+        """One field as an array indexed [k, j, i] (x fastest)."""
+        which = FIELDS.index(name)
+        out = np.empty(self.nNodesTotal)
+        self._s._check(lib().ns_get_field(self._s._h, which, _dptr(out)))
+        return out.reshape(self.NK, self.NJ, self.NI)
+
+    def conserved(self) -> np.ndarray:
+        return np.stack([self.field(n) for n in FIELDS[:5]])
+
+    @property
+    def nodeType(self) -> np.ndarray:
+        out = np.empty(self.nNodesTotal, dtype=np.uint8)
+        self._s._check(lib().ns_get_node_types(self._s._h, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out.reshape(self.NK, self.NJ, self.NI)
+
+    def counts(self):
+        g, f, n = C.c_int64(), C.c_int64(), C.c_int64()
+        self._s._check(lib().ns_get_counts(self._s._h, C.byref(g), C.byref(f), C.byref(n)))
+        return g.value, f.value, n.value
+
+    def ghosts(self):
+        """(ijk, body-intercept, image point, box lower corner) of every ghost, sorted by reference index."""
+        n = self.counts()[0]
+        ijk = np.zeros((n, 3), dtype=np.int32)
+        bi = np.zeros((n, 3))
+        ip = np.zeros((n, 3))
+        box = np.zeros((n, 3), dtype=np.int32)
+        self._s._check(lib().ns_get_ghosts(self._s._h, ijk.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(bi), _dptr(ip),
+                                           box.ctypes.data_as(C.POINTER(C.c_int32))))
+        return ijk, bi, ip, box
+
+    def filterNodes(self):
+        n = self.counts()[1]
+        ijk = np.zeros((n, 3), dtype=np.int32)
+        if n:
+            self._s._check(lib().ns_get_filter_nodes(self._s._h, ijk.ctypes.data_as(C.POINTER(C.c_int32))))
+        return ijk
+
+    def totals(self):
+        out = np.zeros(5)
+        self._s._check(lib().ns_totals(self._s._h, _dptr(out)))
+        return out
+
+
+class Solver:
+    """Solver of the reference (src/Solver.h:17-31) on one MI355X."""
+
+    def __init__(self, params):
+        self.config = params.config if isinstance(params, ConfigSettings) else params
+        self._h = C.c_void_p()
+        s = lib().ns_create(C.byref(self.config), C.byref(self._h))
+        if s != 0:
+            msg = lib().ns_last_error(self._h).decode() if self._h else ""
+            lib().ns_destroy(self._h)
+            self._h = C.c_void_p()
+            raise NSError(s, msg)
+        self.mesh = Mesh(self)
+
+    def _check(self, s):
+        if s != 0:
+            raise NSError(s, lib().ns_last_error(self._h).decode())
+
+    def initialize(self):
+        self._check(lib().ns_init(self._h))
+
+    def setInitialField(self, U5: np.ndarray):
+        a = np.ascontiguousarray(U5, dtype=np.float64)
+        assert a.size == 5 * self.mesh.nNodesTotal
+        self._check(lib().ns_set_initial_field(self._h, _dptr(a)))
+
+    def marchTimeStep(self, nsteps: int = 1):
+        """Advances nsteps RK4 steps; returns (t, timeLevel)."""
+        t, lv, dt = C.c_double(), C.c_uint64(), C.c_double()
+        self._check(lib().ns_step(self._h, int(nsteps), C.byref(t), C.byref(lv), C.byref(dt)))
+        return t.value, lv.value
+
+    def time(self):
+        t, lv, dt = C.c_double(), C.c_uint64(), C.c_double()
+        self._check(lib().ns_get_time(self._h, C.byref(t), C.byref(lv), C.byref(dt)))
+        return t.value, lv.value, dt.value
+
+    @property
+    def dt(self):
+        return self.time()[2]
+
+    def profile(self, nsteps: int):
+        """Mean ms per launch of (stage kernel, primitive pass, boundary kernels, reductions) and launches/step."""
+        ms = np.zeros(4)
+        ln = np.zeros(4, dtype=np.int32)
+        self._check(lib().ns_profile(self._h, int(nsteps), _dptr(ms), ln.ctypes.data_as(C.POINTER(C.c_int32))))
+        return ms, ln
+
+    def getConvergenceHistory(self):
+        out = {}
+        for q, name in enumerate(FIELDS[:5]):
+            cnt = C.c_int64()
+            self._check(lib().ns_norm_history(self._h, q, None, 0, C.byref(cnt)))
+            a = np.zeros(cnt.value)
+            if cnt.value:
+                self._check(lib().ns_norm_history(self._h, q, _dptr(a), cnt.value, C.byref(cnt)))
+            out[name] = a
+        return out
+
+    def close(self):
+        if self._h:
+            lib().ns_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Simulation:
+    """Simulation of the reference (src/Simulation.h:17-32): initialize, march until the stop criterion."""
+
+    def __init__(self, params):
+        self.solver = Solver(params)
+        self.t = 0.0
+        self.timeLevel = 0
+
+    def run(self):
+        self.solver.initialize()
+        t, lv = C.c_double(), C.c_uint64()
+        self.solver._check(lib().ns_run(self.solver._h, C.byref(t), C.byref(lv)))
+        self.t, self.timeLevel = t.value, lv.value
+        return self.t, self.timeLevel

@@ -1,0 +1,128 @@
+"""Throughput of the explicit RK4 time step on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sphere512] [--precision fp32|fp64]
+
+A "step" is one full RK4 time step (4 stages, edge BCs, immersed filter and ghost update, dt and norm
+reductions) of the named workload, with the state resident in HBM. Prints ONE JSON line with the metric
+(M node-updates/s = NI*NJ*NK * steps / wall time of the step loop), a roofline object for the dominant
+kernel (the RK4 stage kernel; algorithmic bytes = 20 field values per node per stage launch, i.e. 80
+per RK4 step, timed live with HIP events on the library's stream) and a CPU baseline (the serial FP64
+oracle on a bounded sample of the same workload, 1 core).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "3d-ns-fsi_amd"))
+
+HBM_PEAK_GBS = 8000.0
+VALUES_PER_NODE_STEP = 80  # 3-register RK4: stage 1 reads 5 writes 10, stages 2-3 read 15 write 10, stage 4 reads 10 writes 5
+
+
+def cpu_baseline(workload, precision_unused=None):
+    """Serial FP64 oracle (1 core) on the same workload at 128^3 for a fixed number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cases
+    import oracle
+
+    N, steps = 128, 8
+    cfg = cases.WORKLOADS[workload](N=N, steps=steps, precision="fp64")
+    o = oracle.Oracle(cfg)
+    o.initialize()
+    t0 = time.perf_counter()
+    o.march(steps)
+    el = time.perf_counter() - t0
+    o.close()
+    return {
+        "value": N ** 3 * steps / el / 1e6,
+        "unit": "M node-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{workload} geometry and physics at {N}^3, {steps} RK4 steps, serial FP64 oracle "
+                  f"(g++ -O3, no FMA), {el:.1f} s; host has {os.cpu_count()} cores",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="sphere512")
+    ap.add_argument("--precision", default=None)
+    ap.add_argument("--N", type=int, default=None, help="override the grid edge")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build's path runs on a single MI355X (no multi-GPU sharding); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import cases
+    import nsfsi
+
+    mk = cases.WORKLOADS[a.workload]
+    kw = {}
+    if a.N:
+        kw["N"] = a.N
+    prec = a.precision or ("fp32" if a.workload == "sphere512" else "fp64")
+    cfg = mk(steps=a.steps + a.warmup, precision=prec, **kw)
+    cfg.conv_history = 2
+    nodes = cfg.NI * cfg.NJ * cfg.NK
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    s.marchTimeStep(a.warmup)
+    t0 = time.perf_counter()
+    s.marchTimeStep(a.steps)  # returns after a device synchronisation
+    el = time.perf_counter() - t0
+    ms_step = el / a.steps * 1e3
+    value = nodes * a.steps / el / 1e6
+
+    # dominant kernel timed live with HIP events on the library's stream
+    ms, launches = s.profile(max(2, min(5, a.steps)))
+    eb = 4 if prec == "fp32" else 8
+    alg_bytes = VALUES_PER_NODE_STEP / 4 * eb * nodes  # per stage launch
+    achieved = alg_bytes / (ms[0] * 1e-3) / 1e9 if ms[0] > 0 else None
+    g, f, _ = s.mesh.counts()
+    line = {
+        "metric": "M node-updates/s per RK4 step",
+        "value": round(value, 3),
+        "unit": "M node-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32" else "f64",
+        "data": "synthetic (seeded freestream + perturbation initial field)",
+        "config": {"workload": a.workload, "grid": [cfg.NI, cfg.NJ, cfg.NK], "precision": prec,
+                   "ghosts": g, "filter_nodes": f, "stage_kernel": "plain (primitive pass + per-node RHS)"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": None,
+            "kernel": "k_rhs_plain (RK4 stage)",
+            "kernel_ms": round(ms[0], 4),
+            "step_breakdown_ms": {"stage_kernel": round(ms[0] * launches[0], 3),
+                                  "primitive_pass": round(ms[1] * launches[1], 3),
+                                  "boundaries": round(ms[2] * launches[2], 3),
+                                  "reductions": round(ms[3] * launches[3], 3)},
+            "step_frac": round(nodes * VALUES_PER_NODE_STEP * eb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+    }
+    s.close()
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(a.workload)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,188 @@
+/*
+ * ns_fsi.h — C-ABI of the MI355X-native explicit compressible Navier–Stokes time step.
+ *
+ * One handle = one simulation case on one GPU, one HIP stream. All device memory is owned by the
+ * handle; host pointers are borrowed only for the duration of a call. Every entry point returns an
+ * ns_status; a handle keeps the text of its last error (ns_last_error). No exception crosses this
+ * boundary. Handles are not thread-safe; calls on one handle must be serialised by the caller.
+ *
+ * The reference (Frederik-Kris/3D-NS-FSI) has no FFI: the path sits behind C++ member calls. Each
+ * entry point below names the reference call it replaces (paths relative to the reference root).
+ * Field arrays crossing this boundary are dense NI*NJ*NK arrays in x-fastest order,
+ * index = (k*NJ + j)*NI + i; the reference stores k fastest, (i*NJ + j)*NK + k (src/Array3D.h:42).
+ */
+#ifndef NS_FSI_H_
+#define NS_FSI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NS_FSI_ABI_VERSION 1
+
+typedef enum ns_status {
+    NS_OK = 0,
+    NS_ERR_CONFIG = 1,       /* ConfigFile parse / type / value error (src/ConfigSettings.cpp:31-163) */
+    NS_ERR_GEOMETRY = 2,     /* body/image-point invariant violated (src/Boundary.cpp:724, :36) */
+    NS_ERR_NONFINITE = 3,    /* non-finite conserved value found (src/Mesh.h:48-61) */
+    NS_ERR_HIP = 4,          /* HIP runtime error, or no gfx950 device */
+    NS_ERR_ARGUMENT = 5,     /* null handle, bad pointer or out-of-range argument */
+    NS_ERR_STATE = 6,        /* call out of order (e.g. step before init) */
+    NS_ERR_UNSUPPORTED = 7   /* configuration the device path does not implement */
+} ns_status;
+
+typedef enum ns_bc_type {
+    NS_BC_INLET = 0, NS_BC_OUTLET = 1, NS_BC_PERIODIC = 2, NS_BC_SYMMETRY = 3
+} ns_bc_type;
+
+/* Face order used by ns_config.bc[]. */
+typedef enum ns_face { NS_XMIN = 0, NS_XMAX = 1, NS_YMIN = 2, NS_YMAX = 3, NS_ZMIN = 4, NS_ZMAX = 5 } ns_face;
+
+typedef enum ns_body_type { NS_BODY_NONE = 0, NS_BODY_CYLINDER = 1, NS_BODY_SPHERE = 2 } ns_body_type;
+
+typedef enum ns_ic_type {
+    NS_IC_STAGNATION = 0,          /* src/Solver.cpp:31-51 (the reference default) */
+    NS_IC_UNIFORM_OBLIQUE = 1,     /* src/Solver.cpp:54-75 */
+    NS_IC_FREESTREAM_GAUSSIAN = 2, /* u = M_0 + N(0, a^2), v = N(0,a^2), w = N(0,a^2), p' = T' = 0 */
+    NS_IC_FREESTREAM_UNIFORM = 3   /* u = M_0 + U[-a,a], v = U[-a,a], w = U[-a,a], p' = T' = 0 */
+} ns_ic_type;
+
+typedef enum ns_precision { NS_FP64 = 0, NS_FP32 = 1 } ns_precision;
+
+typedef enum ns_field {
+    NS_RHO = 0, NS_RHO_U = 1, NS_RHO_V = 2, NS_RHO_W = 3, NS_RHO_E = 4,
+    NS_U = 5, NS_V = 6, NS_W = 7, NS_P = 8, NS_T = 9, NS_MU = 10, NS_KAPPA = 11
+} ns_field;
+
+/* Node classes; values equal the reference enum order (src/Node.h:27-34). */
+typedef enum ns_node_type {
+    NS_FLUID_INTERIOR = 0, NS_FLUID_EDGE = 1, NS_FLUID_GHOST = 2, NS_SOLID_INACTIVE = 3, NS_SOLID_GHOST = 4
+} ns_node_type;
+
+#define NS_MAX_FILTER 16
+
+/* Every ConfigFile key of the reference (ConfigFile:22-125, src/ConfigSettings.h:39-74) plus the
+ * extension keys for what the reference hard-codes (src/Mesh.cpp:33-57, src/Solver.cpp:26,
+ * src/Boundary.cpp:267). Fill with ns_config_default(), then override. */
+typedef struct ns_config {
+    int32_t NI, NJ, NK;
+    int32_t stop_criterion;            /* 0 = "timesteps", 1 = "end_time" */
+    uint64_t stop_time_level;
+    double t_end;
+    double conv_stability_limit;
+    double visc_stability_limit;
+    int32_t n_filter_intervals;
+    uint32_t filter_intervals[NS_MAX_FILTER];
+    int32_t n_filter_change_times;
+    double filter_change_times[NS_MAX_FILTER];
+    double status_report_interval;
+    int32_t save_density, save_momentum, save_energy, save_velocity;
+    int32_t save_pressure, save_temperature, save_viscosity, save_thermal_cond;
+    int32_t save_ic, save_final, save_intervals;
+    double save_period, save_intervals_start_time, save_intervals_end_time;
+    int32_t conv_history;              /* 0 = "none", 1 = "density", 2 = "all" */
+    double gamma, Pr, R, Re, sutherlands_C2, M_0, T_0, L_x, L_y, L_z;
+    /* derived: machine-precision buffer eps*(L_x+L_y+L_z) (src/ConfigSettings.cpp:160-163) */
+    double machine_precision_buffer;
+    /* ---- extension keys (defaults reproduce the reference's hard-coded layout) ---- */
+    int32_t bc[6];                     /* ns_bc_type per ns_face */
+    int32_t body_type;                 /* ns_body_type */
+    int32_t body_axis;                 /* 0,1,2 = x,y,z (cylinder only) */
+    double body_centre[3];
+    double body_radius;
+    int32_t ic_type;                   /* ns_ic_type */
+    uint64_t ic_seed;
+    double ic_amplitude;
+    double inlet_ramp_time;            /* reference: 10 (src/Boundary.cpp:267) */
+    int32_t precision;                 /* ns_precision */
+    int32_t kahan;                     /* FP32 only: compensated stage combine */
+} ns_config;
+
+typedef struct ns_handle ns_handle;
+
+/* Defaults: the reference's shipped ConfigFile and its hard-coded geometry. */
+void ns_config_default(ns_config* cfg);
+
+/* Parse libconfig-syntax text (the reference ConfigFile format) into cfg. Replaces
+ * ConfigSettings::loadSettings (src/ConfigSettings.cpp:18-28). On NS_ERR_CONFIG, errbuf (if not
+ * null) receives the message. */
+ns_status ns_config_parse(const char* text, ns_config* cfg, char* errbuf, size_t errbuf_len);
+
+/* Replaces Solver::Solver / Mesh::Mesh (src/Solver.cpp:13-18, src/Mesh.cpp:10-21). Allocates the
+ * device fields. Fails with NS_ERR_HIP when no gfx950 device is present. */
+ns_status ns_create(const ns_config* cfg, ns_handle** out);
+
+/* Replaces Solver::initialize (src/Solver.cpp:22-28): boundaries, node classification (device
+ * kernel + compaction), ghost geometry, initial condition, first dt. */
+ns_status ns_init(ns_handle* h);
+
+/* Replaces the IC upload of applyStagnation_IC (src/Solver.cpp:31-51) with an arbitrary field:
+ * U5 holds rho', rho*u, rho*v, rho*w, rho*E' as 5 consecutive x-fastest NI*NJ*NK arrays. Primitive
+ * and transport fields are derived everywhere; dt is recomputed. Call after ns_init, before steps. */
+ns_status ns_set_initial_field(ns_handle* h, const double* U5);
+
+/* Replaces nsteps calls of Solver::marchTimeStep(t, timeLevel) (src/Solver.cpp:130-167). t, level
+ * and dt (any may be null) receive the values after the last step; dt is the size of the next step
+ * (public member Solver::dt, src/Solver.h:31). */
+ns_status ns_step(ns_handle* h, int32_t nsteps, double* t, uint64_t* level, double* dt);
+
+/* Runs Simulation::run's loop (src/Simulation.cpp:19-52) until the stopping criterion holds. */
+ns_status ns_run(ns_handle* h, double* t, uint64_t* level);
+
+/* Current t, timeLevel and dt without stepping. */
+ns_status ns_get_time(ns_handle* h, double* t, uint64_t* level, double* dt);
+
+/* Copy one field (ns_field) to host, x-fastest NI*NJ*NK doubles. The conserved fields are the
+ * state Mesh::conservedVariables; primitives and transport are the values the reference keeps in
+ * Mesh::primitiveVariables / transportProperties (src/Mesh.h:82-85). */
+ns_status ns_get_field(ns_handle* h, int32_t which, double* out);
+
+/* Node classes (ns_node_type), x-fastest NI*NJ*NK bytes (Mesh::nodeType, src/Mesh.h:88). */
+ns_status ns_get_node_types(ns_handle* h, uint8_t* out);
+
+/* Number of solid ghost nodes and filter nodes of the immersed body (src/Boundary.h:200-210). */
+ns_status ns_get_counts(ns_handle* h, int64_t* n_ghosts, int64_t* n_filter, int64_t* n_interior);
+
+/* Ghost nodes (GhostNode, src/Node.h:15-22), sorted by reference index (i*NJ+j)*NK+k:
+ * ijk[3*n] node indices, bi[3*n] body intercept, ip[3*n] image point, box[3*n] lower corner of the
+ * 8-node image-point box (src/SmallVectors.h:108-127). Any pointer may be null. */
+ns_status ns_get_ghosts(ns_handle* h, int32_t* ijk, double* bi, double* ip, int32_t* box);
+
+/* Filter nodes (src/Boundary.cpp:1010-1012, :1099-1101), sorted by reference index, ijk[3*n]. */
+ns_status ns_get_filter_nodes(ns_handle* h, int32_t* ijk);
+
+/* Norm-of-change history (Solver::getConvergenceHistory, src/Solver.h:27; src/Mesh.cpp:171-178):
+ * var 0..4 = rho..rho_E; copies min(n, available) values, *count receives the number available. */
+ns_status ns_norm_history(ns_handle* h, int32_t var, double* out, int64_t n, int64_t* count);
+
+/* FP64 totals over FluidInterior nodes of the current state: sum of each conserved variable times
+ * dx*dy*dz (mass, momenta, energy). No reference counterpart; conservation monitor. */
+ns_status ns_totals(ns_handle* h, double out[5]);
+
+/* Grid spacing and origin offset (Mesh::dx/dy/dz, positionOffset; src/Mesh.cpp:44-49). */
+ns_status ns_get_geometry(ns_handle* h, double spacing[3], double offset[3]);
+
+/* Measurement hook (no reference counterpart): runs nsteps time steps with HIP events recorded on the
+ * handle's stream around each kernel group and returns the mean duration in milliseconds of one launch
+ * of each group: ms[0] the RK4 stage kernel, ms[1] the primitive-variable pass (0 when fused),
+ * ms[2] the boundary kernels of one stage, ms[3] the end-of-step reductions; launches[g] = launches per
+ * step. The steps advance the solution like ns_step. */
+ns_status ns_profile(ns_handle* h, int32_t nsteps, double ms[4], int32_t launches[4]);
+
+/* Last error text of this handle (never null). */
+const char* ns_last_error(ns_handle* h);
+
+/* Releases all device memory. Null is accepted. */
+void ns_destroy(ns_handle* h);
+
+/* Library ABI version, NS_FSI_ABI_VERSION. */
+int32_t ns_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NS_FSI_H_ */
