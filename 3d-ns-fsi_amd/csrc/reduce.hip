@@ -8,6 +8,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.h"
 #include "stage.h"
 
@@ -104,7 +106,7 @@ __global__ void k_inactive_reduce(Layout L, Params P, const T* A, const int64_t*
     __shared__ double sh[8];
     const double vm = fmax(4. / 3, P.gamma / P.Pr);
     double mx[4] = {0, 0, 0, 0};
-    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         int64_t m = nodes[q];
         int i = (int)(m % L.NI);
         int64_t r = m / L.NI;
@@ -117,12 +119,13 @@ __global__ void k_inactive_reduce(Layout L, Params P, const T* A, const int64_t*
         mx[2] = fmax(mx[2], cs + fabs(ic[2]));
         mx[3] = fmax(mx[3], ic[4] / (rho + 1) * vm);
     }
+    double* out = partial + (size_t)blockIdx.x * NP;
     for (int q = 0; q < 4; ++q) {
         double v = blockMax(mx[q], sh);
-        if (threadIdx.x == 0) partial[q] = v;
+        if (threadIdx.x == 0) out[q] = v;
     }
     if (threadIdx.x == 0)
-        for (int q = 4; q < NP; ++q) partial[q] = 0;
+        for (int q = 4; q < NP; ++q) out[q] = 0;
 }
 
 // Combines the partials and advances t, timeLevel, dt. mode 0: initial dt only (Solver::initialize);
@@ -175,7 +178,8 @@ void launchStepReduce(hipStream_t s, const Layout& L, const Params& P, const T* 
 template <class T>
 void launchInactiveReduce(hipStream_t s, const Layout& L, const Params& P, const T* A, const int64_t* nodes,
                           const double* icPrim, int64_t n, double* partial) {
-    hipLaunchKernelGGL(k_inactive_reduce<T>, dim3(1), dim3(RB), 0, s, L, P, A, nodes, icPrim, n, partial);
+    hipLaunchKernelGGL(k_inactive_reduce<T>, dim3(inactiveReduceBlocks(n)), dim3(RB), 0, s, L, P, A, nodes, icPrim, n,
+                       partial);
 }
 
 void launchFinalize(hipStream_t s, const Params& P, const double* partials, int nPartials, int mode, int convHistory,
@@ -185,6 +189,8 @@ void launchFinalize(hipStream_t s, const Params& P, const double* partials, int 
 }
 
 int reducePartialSlots() { return NP; }
+
+int inactiveReduceBlocks(int64_t n) { return (int)std::min<int64_t>(64, std::max<int64_t>(1, (n + RB * 8 - 1) / (RB * 8))); }
 
 template void launchStepReduce<double>(hipStream_t, const Layout&, const Params&, const double*, const double*,
                                        const uint8_t*, const Window&, const double*, const double*, int, double*, int);

@@ -314,7 +314,7 @@ void Case::allocFields() {
     rhoEPre = alloc(wn * sizeof(T));
     ftmp = alloc(2 * std::max<int64_t>(1, nFilter) * sizeof(T));
     nPartials = 1024;
-    partials = (double*)alloc((size_t)(nPartials + 1) * reducePartialSlots() * sizeof(double));
+    partials = (double*)alloc((size_t)(nPartials + 64) * reducePartialSlots() * sizeof(double));
     scal = (StepScalars*)alloc(sizeof(StepScalars));
     hist = (double*)alloc((size_t)histCap * 5 * sizeof(double));
     StepScalars s0{};
@@ -401,7 +401,7 @@ void Case::computeDt(int mode) {
     if (nInactive > 0) {
         launchInactiveReduce<T>(stream, L, P, mode == 0 ? (const T*)U : (const T*)A, inactiveList, inactiveIC, nInactive,
                                 partials + (size_t)nPartials * reducePartialSlots());
-        npart += 1;
+        npart += inactiveReduceBlocks(nInactive);
     }
     launchFinalize(stream, P, partials, npart, mode, cfg.conv_history, cfg.stop_criterion == 1, cfg.t_end,
                    cfg.conv_stability_limit, cfg.visc_stability_limit, scal, hist, histCap);

@@ -40,5 +40,6 @@ void launchInactiveReduce(hipStream_t s, const Layout& L, const Params& P, const
 void launchFinalize(hipStream_t s, const Params& P, const double* partials, int nPartials, int mode, int convHistory,
                     int stopEndTime, double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap);
 int reducePartialSlots();
+int inactiveReduceBlocks(int64_t n);  // partial slots the inactive-node reduction writes (<= 64)
 
 }  // namespace nsfsi
