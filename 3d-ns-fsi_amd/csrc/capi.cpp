@@ -205,6 +205,21 @@ ns_status ns_profile(ns_handle* h, int32_t nsteps, double ms[4], int32_t launche
     });
 }
 
+ns_status ns_set_option(ns_handle* h, const char* name, int64_t value) {
+    return guarded(h, [&] {
+        if (!name) throw std::invalid_argument("null option name");
+        live(h).setOption(name, value);
+    });
+}
+
+ns_status ns_get_tile(ns_handle* h, int32_t* ty, int32_t* kc) {
+    return guarded(h, [&] {
+        auto& c = live(h);
+        if (ty) *ty = c.tile.ty;
+        if (kc) *kc = c.tile.kc;
+    });
+}
+
 const char* ns_last_error(ns_handle* h) {
     if (!h) return "null handle";
     return h->err.c_str();

@@ -46,6 +46,14 @@ public:
     // one launch of group g (0 stage kernel, 1 primitive pass, 2 boundary kernels of one stage,
     // 3 end-of-step reductions) and launches[g] the launches per step.
     void profile(int nsteps, double ms[4], int launches[4]);
+    // Run-time options: "stage_kernel" 0 = unfused (primitive pass + per-node RHS), 1 = fused.
+    void setOption(const std::string& name, int64_t value);
+    int stageKernel = 1;
+    FusedTile tile{};
+    // Times every fused-stage tile shape on this case (stage-2 launches into scratch outputs, state
+    // untouched) and keeps the fastest; returns the chosen tile and its time per launch in ms.
+    double autotune(int reps);
+    std::vector<double> tuneTimes;  // ms per launch of each tried shape, in try order
 
     ns_config cfg;
     bool fp32 = false;

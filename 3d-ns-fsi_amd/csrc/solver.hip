@@ -433,12 +433,19 @@ void Case::stepOnce() {
     T* out[4] = {s1, s2, s1, a};
     const double frac[4] = {0.5, 0.5, 1.0, 1.0};
     for (int st = 1; st <= 4; ++st) {
-        mark(1, true);
-        launchDerive<T>(stream, L, P, in[st - 1], pr, cls, W, (const T*)rhoPre, (const T*)rhoEPre);
-        mark(1, false);
-        mark(0, true);
-        launchRhsPlain<T>(stream, st, L, P, in[st - 1], pr, u, out[st - 1], a, cls, dtdev);
-        mark(0, false);
+        if (stageKernel == 1) {
+            mark(0, true);
+            launchStageFused<T>(stream, st, L, P, W, in[st - 1], u, out[st - 1], a, cls, (const T*)rhoPre,
+                                (const T*)rhoEPre, dtdev, tile);
+            mark(0, false);
+        } else {
+            mark(1, true);
+            launchDerive<T>(stream, L, P, in[st - 1], pr, cls, W, (const T*)rhoPre, (const T*)rhoEPre);
+            mark(1, false);
+            mark(0, true);
+            launchRhsPlain<T>(stream, st, L, P, in[st - 1], pr, u, out[st - 1], a, cls, dtdev);
+            mark(0, false);
+        }
         mark(2, true);
         applyBCs<T>(out[st - 1], frac[st - 1]);
         mark(2, false);
@@ -447,6 +454,74 @@ void Case::stepOnce() {
     computeDt<T>(1);
     mark(3, false);
     std::swap(U, A);
+}
+
+void Case::setOption(const std::string& name, int64_t value) {
+    if (name == "stage_kernel") {
+        if (value != 0 && value != 1) throw std::invalid_argument("stage_kernel must be 0 or 1");
+        stageKernel = (int)value;
+    } else if (name == "tile_ty") {
+        int n = 0;
+        const int* v = fusedTileHeights(&n);
+        bool ok = false;
+        for (int q = 0; q < n; ++q) ok = ok || v[q] == value;
+        if (!ok) throw std::invalid_argument("tile_ty must be one of the compiled tile heights");
+        tile.ty = (int)value;
+    } else if (name == "tile_kc") {
+        if (value < 2 || value > 4096) throw std::invalid_argument("tile_kc out of range");
+        tile.kc = (int)value;
+    } else if (name == "autotune") {
+        if (!initialized) throw StateError("autotune before init");
+        autotune(value > 0 ? (int)value : 3);
+    } else {
+        throw std::invalid_argument("unknown option " + name);
+    }
+}
+
+double Case::autotune(int reps) {
+    // stage-2 launches (reads U, A; writes S2 and the interior of A): S2 is rewritten by every step and
+    // A's interior by every stage 1, so the state of the run is unchanged
+    int n = 0;
+    const int* tys = fusedTileHeights(&n);
+    const int kcs[4] = {16, 32, 64, 128};
+    hipEvent_t e0, e1;
+    hipCheck(hipEventCreate(&e0), "event");
+    hipCheck(hipEventCreate(&e1), "event");
+    double best = 1e30;
+    FusedTile bestTile = tile;
+    tuneTimes.clear();
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < 4; ++b) {
+            FusedTile tl;
+            tl.ty = tys[a];
+            tl.kc = kcs[b];
+            auto run = [&](int count) {
+                for (int r = 0; r < count; ++r) {
+                    if (fp32)
+                        launchStageFused<float>(stream, 2, L, P, W, (const float*)U, (const float*)U, (float*)S2, (float*)A,
+                                                cls, (const float*)rhoPre, (const float*)rhoEPre, &scal->dt, tl);
+                    else
+                        launchStageFused<double>(stream, 2, L, P, W, (const double*)U, (const double*)U, (double*)S2,
+                                                 (double*)A, cls, (const double*)rhoPre, (const double*)rhoEPre,
+                                                 &scal->dt, tl);
+                }
+            };
+            run(1);  // warm-up
+            hipCheck(hipEventRecord(e0, stream), "record");
+            run(reps);
+            hipCheck(hipEventRecord(e1, stream), "record");
+            hipCheck(hipEventSynchronize(e1), "sync");
+            float ms = 0;
+            hipCheck(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            double per = ms / reps;
+            tuneTimes.push_back(per);
+            if (per < best) { best = per; bestTile = tl; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hipCheck(hipGetLastError(), "autotune");
+    tile = bestTile;
+    return best;
 }
 
 void Case::mark(int group, bool begin) {

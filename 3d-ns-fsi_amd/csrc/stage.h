@@ -30,6 +30,19 @@ template <class T>
 void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                     T* Sout, T* A, const uint8_t* cls, const double* dtdev);
 
+// ---- fused stage (stage_fused.hip): primitives + RHS + combine in one pass
+// Tile shape of the fused stage: ty output rows per block (one of fusedTileHeights), kc z-planes per
+// block (the z-chunk each block streams through).
+struct FusedTile {
+    int ty = 8;
+    int kc = 64;
+};
+template <class T>
+void launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
+                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
+                      const double* dtdev, FusedTile tile);
+const int* fusedTileHeights(int* n);
+
 // ---- end-of-step reductions (reduce.hip)
 template <class T>
 void launchStepReduce(hipStream_t s, const Layout& L, const Params& P, const T* U, const T* A, const uint8_t* cls,

@@ -1,0 +1,435 @@
+/*
+ * stage_fused.hip — the fused RK4 stage: one kernel computes primitive variables, the 19-point
+ * convective + viscous + heat right-hand side (compute_RK4_step_*, src/Solver.cpp:181-418) and the
+ * stage combine (src/Solver.cpp:422-507) in one pass over the grid.
+ *
+ * Flux form. The reference's per-node expression is regrouped into face fluxes: for every direction d,
+ *   RHS(P) = sum_d [ Phi_d(P + h_d/2) - Phi_d(P - h_d/2) ],
+ *   Phi_d(a, b) = (G_d(a) + G_d(b)) / (2 h_d) + C_d(a, b),
+ * where G_d is the nodal central flux (inviscid flux plus the cross-derivative viscous stresses, e.g.
+ * G_x of x-momentum = -(rho u u + p) - 2/3 mu (dv/dy + dw/dz)) and C_d the compact second-difference
+ * terms, e.g. 2/(3 dx^2) (mu_a + mu_b)(u_b - u_a). Expanding the differences gives back every term of
+ * the reference expression (same stencil, same coefficients); only the rounding differs. Each face flux
+ * is computed once and shared by the two nodes of the face.
+ *
+ * Tiling. A block is 64 x (TY + 2) threads: one wavefront per x-row. Lanes 1..62 own output columns,
+ * lanes 0 and 63 are the x-halo; rows 1..TY own output rows, rows 0 and TY+1 the y-halo. The block
+ * streams through a chunk of z-planes: the own column's neighbours in z live in registers, x-neighbours
+ * come by cross-lane shuffle, y-neighbours through LDS. Primitive variables are recomputed from the
+ * conserved state; at immersed-filter nodes from the pre-filter density and energy (the reference keeps
+ * primitives pre-filter there). Blocks are mapped so that each XCD sweeps a contiguous slab (its L2 then
+ * holds the halo of its neighbours).
+ *
+ * Multiply-adds are fused only inside one source expression (contract(on), not the compiler's
+ * cross-statement fusion), so the arithmetic of a node does not depend on which block, tile shape or
+ * z-chunk computes it: results are bit-identical for every tile shape.
+ */
+#pragma clang fp contract(on)
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "stage.h"
+
+namespace nsfsi {
+
+namespace {
+
+template <class T>
+struct Coef {
+    T i2dx, i2dy, i2dz;      // 1 / (2 h)
+    T c43x, c43y, c43z;      // 2 / (3 h^2)
+    T c1x, c1y, c1z;         // 1 / (2 h^2)
+    T gamma, gm1, H0, ScP1, Re, pf;
+};
+
+template <class T>
+struct FusedArgs {
+    Layout L;
+    Window W;
+    Coef<T> c;
+    const T* Sin;
+    const T* U;
+    T* Sout;
+    T* A;
+    const uint8_t* cls;
+    const T* rhoPre;
+    const T* rhoEPre;
+    const double* dtdev;
+    int nbx, nby, nbz, kc;   // tiles in x, y, z and planes per z-chunk
+    int nblocks;
+};
+
+template <class T>
+__device__ __forceinline__ T shl1(T x) { return __shfl_down(x, 1, 64); }  // value of lane + 1
+template <class T>
+__device__ __forceinline__ T shr1(T x) { return __shfl_up(x, 1, 64); }    // value of lane - 1
+
+// Point state at one plane of the own column.
+template <class T>
+struct Pt {
+    T ru, rv, rw, E;      // conserved (current buffer values)
+    T rho;                // current density (for the combine / copies)
+    T u, v, w, p, Tt, mu; // primitives (pre-filter at filter nodes)
+    T dxu, dxv, dxw, dyu, dyv, dyw;  // in-plane central derivatives
+};
+
+template <class T>
+__device__ __forceinline__ void prims(const Coef<T>& c, T rho, T ru, T rv, T rw, T E, Pt<T>& q) {
+    T m = (T)1 + rho;
+    T im = (T)1 / m;
+    q.u = ru * im;
+    q.v = rv * im;
+    q.w = rw * im;
+    q.p = c.gm1 * (E - m * (T)0.5 * (q.u * q.u + q.v * q.v + q.w * q.w));
+    q.Tt = (c.gamma * q.p - rho) * im;
+    T a = (T)1 + q.Tt;
+    q.mu = a * sqrt(a) * c.ScP1 / (c.Re * (q.Tt + c.ScP1));
+}
+
+// Compact face terms between a and b along one direction; k43 is the coefficient of the velocity
+// component normal to the face (4/3 mu), k1 of the tangential ones and of the heat flux.
+template <class T, int DIR>
+__device__ __forceinline__ void compact(const Coef<T>& c, const Pt<T>& a, const Pt<T>& b, T out[5]) {
+    const T k43 = DIR == 0 ? c.c43x : (DIR == 1 ? c.c43y : c.c43z);
+    const T k1 = DIR == 0 ? c.c1x : (DIR == 1 ? c.c1y : c.c1z);
+    T mm = a.mu + b.mu;
+    T du = b.u - a.u, dv = b.v - a.v, dw = b.w - a.w, dT = b.Tt - a.Tt;
+    T ku = DIR == 0 ? k43 : k1, kv = DIR == 1 ? k43 : k1, kw = DIR == 2 ? k43 : k1;
+    out[0] = 0;
+    out[1] = ku * mm * du;
+    out[2] = kv * mm * dv;
+    out[3] = kw * mm * dw;
+    out[4] = ku * (b.mu * b.u + a.mu * a.u) * du + kv * (b.mu * b.v + a.mu * a.v) * dv +
+             kw * (b.mu * b.w + a.mu * a.w) * dw + k1 * c.pf * mm * dT;
+}
+
+// Nodal central fluxes. dzu/dzv/dzw are the z-derivatives at the point (from the own column).
+template <class T>
+__device__ __forceinline__ void gx(const Coef<T>& c, const Pt<T>& q, T dzu, T dzw, T g[5]) {
+    const T h = c.H0 + q.E + q.p;
+    T t0 = -(T)(2.0 / 3.0) * q.mu * (q.dyv + dzw), t1 = q.mu * q.dyu, t2 = q.mu * dzu;
+    g[0] = -q.ru;
+    g[1] = -(q.ru * q.u + q.p) + t0;
+    g[2] = -q.rv * q.u + t1;
+    g[3] = -q.rw * q.u + t2;
+    g[4] = -h * q.u + q.u * t0 + q.v * t1 + q.w * t2;
+}
+template <class T>
+__device__ __forceinline__ void gy(const Coef<T>& c, const Pt<T>& q, T dzv, T dzw, T g[5]) {
+    const T h = c.H0 + q.E + q.p;
+    T t0 = q.mu * q.dxv, t1 = -(T)(2.0 / 3.0) * q.mu * (q.dxu + dzw), t2 = q.mu * dzv;
+    g[0] = -q.rv;
+    g[1] = -q.ru * q.v + t0;
+    g[2] = -(q.rv * q.v + q.p) + t1;
+    g[3] = -q.rw * q.v + t2;
+    g[4] = -h * q.v + q.u * t0 + q.v * t1 + q.w * t2;
+}
+template <class T>
+__device__ __forceinline__ void gz(const Coef<T>& c, const Pt<T>& q, T g[5]) {
+    const T h = c.H0 + q.E + q.p;
+    T t0 = q.mu * q.dxw, t1 = q.mu * q.dyw, t2 = -(T)(2.0 / 3.0) * q.mu * (q.dxu + q.dyv);
+    g[0] = -q.rw;
+    g[1] = -q.ru * q.w + t0;
+    g[2] = -q.rv * q.w + t1;
+    g[3] = -(q.rw * q.w + q.p) + t2;
+    g[4] = -h * q.w + q.u * t0 + q.v * t1 + q.w * t2;
+}
+
+// LDS layout: [component][row][lane]
+template <class T, int TY>
+struct Smem {
+    T uvw[3][TY + 2][64];
+    T face[10][TY + 2][64];
+};
+
+// Raw conserved values of one point, fetched ahead of use (software pipelining).
+template <class T>
+struct Raw {
+    T rho, ru, rv, rw, E;  // current buffer values
+    T rp, Ep;              // density and energy for the primitives (pre-filter at filter nodes)
+};
+
+template <class T>
+__device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, int i, int j, int k) {
+    const Layout& L = a.L;
+    const int ic = min(max(i, 0), L.NI - 1), jc = min(max(j, 0), L.NJ - 1), kc = min(max(k, 0), L.NK - 1);
+    const size_t o = L.off(ic, jc, kc);
+    const size_t F = L.fstride;
+    Raw<T> r;
+    r.rho = a.Sin[o];
+    r.ru = a.Sin[o + F];
+    r.rv = a.Sin[o + 2 * F];
+    r.rw = a.Sin[o + 3 * F];
+    r.E = a.Sin[o + 4 * F];
+    r.rp = r.rho;
+    r.Ep = r.E;
+    if (a.W.contains(ic, jc, kc) && (a.cls[o] & CLS_FILTER)) {
+        size_t w = a.W.idx(ic, jc, kc);
+        r.rp = a.rhoPre[w];
+        r.Ep = a.rhoEPre[w];
+    }
+    return r;
+}
+
+// Primitives and in-plane derivatives of a fetched plane; x-derivatives by shuffle, y through LDS.
+template <class T, int TY>
+__device__ __forceinline__ void finishPlane(const FusedArgs<T>& a, Smem<T, TY>& sm, int row, int lane, const Raw<T>& r,
+                                            Pt<T>& q) {
+    q.rho = r.rho;
+    q.ru = r.ru;
+    q.rv = r.rv;
+    q.rw = r.rw;
+    q.E = r.E;
+    prims(a.c, r.rp, r.ru, r.rv, r.rw, r.Ep, q);
+    T uE = shl1(q.u), uW = shr1(q.u), vE = shl1(q.v), vW = shr1(q.v), wE = shl1(q.w), wW = shr1(q.w);
+    q.dxu = (uE - uW) * a.c.i2dx;
+    q.dxv = (vE - vW) * a.c.i2dx;
+    q.dxw = (wE - wW) * a.c.i2dx;
+    sm.uvw[0][row][lane] = q.u;
+    sm.uvw[1][row][lane] = q.v;
+    sm.uvw[2][row][lane] = q.w;
+    __syncthreads();
+    const int rn = min(row + 1, TY + 1), rs = max(row - 1, 0);
+    q.dyu = (sm.uvw[0][rn][lane] - sm.uvw[0][rs][lane]) * a.c.i2dy;
+    q.dyv = (sm.uvw[1][rn][lane] - sm.uvw[1][rs][lane]) * a.c.i2dy;
+    q.dyw = (sm.uvw[2][rn][lane] - sm.uvw[2][rs][lane]) * a.c.i2dy;
+}
+
+template <class T, int TY>
+__device__ __forceinline__ void readFaceNb(const Smem<T, TY>& sm, int r, int lane, Pt<T>& nb, T g[5]) {
+    for (int q = 0; q < 5; ++q) g[q] = sm.face[q][r][lane];
+    nb.mu = sm.face[5][r][lane];
+    nb.u = sm.face[6][r][lane];
+    nb.v = sm.face[7][r][lane];
+    nb.w = sm.face[8][r][lane];
+    nb.Tt = sm.face[9][r][lane];
+}
+
+template <class T, int STAGE, int TY>
+__global__ __launch_bounds__(64 * (TY + 2)) void k_stage_fused(FusedArgs<T> a) {
+    __shared__ Smem<T, TY> sm;
+    // XCD-aware mapping: hardware block p runs on XCD p % 8; give each XCD a contiguous range of tiles
+    const int p = blockIdx.x;
+    const int per = (a.nblocks + 7) / 8;
+    const int lb = (p % 8) * per + p / 8;
+    if (lb >= a.nblocks) return;
+    const int bx = lb % a.nbx;
+    const int by = (lb / a.nbx) % a.nby;
+    const int bz = lb / (a.nbx * a.nby);
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const Layout& L = a.L;
+    const size_t F = L.fstride;
+    const int i = bx * 62 + lane - 1;
+    const int j = by * TY + row - 1;
+    const int kb = bz * a.kc, ke = min(L.NK, kb + a.kc);
+    const bool outRow = row >= 1 && row <= TY && j < L.NJ;
+    const bool outLane = lane >= 1 && lane <= 62 && i < L.NI;
+    const Coef<T>& c = a.c;
+    const double dtd = *a.dtdev;
+
+    // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
+    Pt<T> qm, q0, q1;
+    Raw<T> rnext = fetch(a, i, j, kb - 1);
+    finishPlane<T, TY>(a, sm, row, lane, rnext, qm);
+    rnext = fetch(a, i, j, kb);
+    __syncthreads();
+    finishPlane<T, TY>(a, sm, row, lane, rnext, q0);
+    rnext = fetch(a, i, j, kb + 1);
+    T gz0[5], phizm[5];
+    {
+        T gzm[5], cz[5];
+        gz(c, qm, gzm);
+        gz(c, q0, gz0);
+        compact<T, 2>(c, qm, q0, cz);
+        for (int q = 0; q < 5; ++q) phizm[q] = (gzm[q] + gz0[q]) * c.i2dz + cz[q];
+    }
+    T umz = qm.u, vmz = qm.v, wmz = qm.w;
+    __syncthreads();  // the prologue's readers of sm.uvw are done before the loop rewrites it
+
+    for (int k = kb; k < ke; ++k) {
+        const Raw<T> rcur = rnext;
+        rnext = fetch(a, i, j, k + 2);
+        // combine inputs of the output node, fetched before the stencil work
+        const bool doOut = outRow && outLane;
+        const size_t o = doOut ? L.off(i, j, k) : 0;
+        bool interior = false;
+        T base[5], acc[5];
+        if (doOut) {
+            interior = i >= 1 && i <= L.NI - 2 && j >= 1 && j <= L.NJ - 2 && k >= 1 && k <= L.NK - 2;
+            if (interior && a.W.contains(i, j, k)) interior = (a.cls[o] & CLS_TYPE_MASK) == NS_FLUID_INTERIOR;
+            if (STAGE == 2 || STAGE == 3) {
+                if (interior)
+                    for (int q = 0; q < 5; ++q) base[q] = a.U[o + q * F];
+            }
+            if (STAGE >= 2) {
+                if (interior)
+                    for (int q = 0; q < 5; ++q) acc[q] = a.A[o + q * F];
+            } else if (!interior) {
+                for (int q = 0; q < 5; ++q) acc[q] = a.A[o + q * F];
+            }
+        }
+        // sm.uvw is rewritten here: every reader of the previous plane's uvw has passed the face barrier
+        finishPlane<T, TY>(a, sm, row, lane, rcur, q1);
+        // z-face above plane k
+        T gz1[5], phiz[5];
+        gz(c, q1, gz1);
+        {
+            T cz[5];
+            compact<T, 2>(c, q0, q1, cz);
+            for (int q = 0; q < 5; ++q) phiz[q] = (gz0[q] + gz1[q]) * c.i2dz + cz[q];
+        }
+        const T dzu = (q1.u - umz) * c.i2dz, dzv = (q1.v - vmz) * c.i2dz, dzw = (q1.w - wmz) * c.i2dz;
+        // y-faces: every row publishes G_y and its compact inputs; output rows evaluate both faces
+        T g[5];
+        gy(c, q0, dzv, dzw, g);
+        for (int q = 0; q < 5; ++q) sm.face[q][row][lane] = g[q];
+        sm.face[5][row][lane] = q0.mu;
+        sm.face[6][row][lane] = q0.u;
+        sm.face[7][row][lane] = q0.v;
+        sm.face[8][row][lane] = q0.w;
+        sm.face[9][row][lane] = q0.Tt;
+        __syncthreads();
+        if (outRow) {
+            T rhs[5];
+            {
+                Pt<T> nb;
+                T gn[5], cy[5];
+                readFaceNb<T, TY>(sm, row + 1, lane, nb, gn);
+                compact<T, 1>(c, q0, nb, cy);
+                for (int q = 0; q < 5; ++q) rhs[q] = (g[q] + gn[q]) * c.i2dy + cy[q];
+                readFaceNb<T, TY>(sm, row - 1, lane, nb, gn);
+                compact<T, 1>(c, nb, q0, cy);
+                for (int q = 0; q < 5; ++q) rhs[q] -= (gn[q] + g[q]) * c.i2dy + cy[q];
+            }
+            // x part: faces between lanes by shuffle
+            T gxv[5];
+            gx(c, q0, dzu, dzw, gxv);
+            Pt<T> e;
+            e.mu = shl1(q0.mu);
+            e.u = shl1(q0.u);
+            e.v = shl1(q0.v);
+            e.w = shl1(q0.w);
+            e.Tt = shl1(q0.Tt);
+            T cx[5];
+            compact<T, 0>(c, q0, e, cx);
+            for (int q = 0; q < 5; ++q) {
+                T phix = (gxv[q] + shl1(gxv[q])) * c.i2dx + cx[q];
+                rhs[q] += phix - shr1(phix);
+                rhs[q] += phiz[q] - phizm[q];
+            }
+            if (doOut) {
+                const T cur[5] = {q0.rho, q0.ru, q0.rv, q0.rw, q0.E};
+                if (interior) {
+                    if (STAGE == 1) {
+                        const T h2 = (T)(dtd / 2), h6 = (T)(dtd / 6);
+                        for (int q = 0; q < 5; ++q) {
+                            a.Sout[o + q * F] = cur[q] + h2 * rhs[q];
+                            a.A[o + q * F] = cur[q] + h6 * rhs[q];
+                        }
+                    } else if (STAGE == 2 || STAGE == 3) {
+                        const T hs = (T)(STAGE == 2 ? dtd / 2 : dtd), h3 = (T)(dtd / 3);
+                        for (int q = 0; q < 5; ++q) {
+                            a.Sout[o + q * F] = base[q] + hs * rhs[q];
+                            a.A[o + q * F] = acc[q] + h3 * rhs[q];
+                        }
+                    } else {
+                        const T h6 = (T)(dtd / 6);
+                        for (int q = 0; q < 5; ++q) a.A[o + q * F] = acc[q] + h6 * rhs[q];
+                    }
+                } else {
+                    if (STAGE == 1)
+                        for (int q = 0; q < 5; ++q) a.Sout[o + q * F] = acc[q];
+                    else if (STAGE == 4)
+                        for (int q = 0; q < 5; ++q) a.A[o + q * F] = cur[q];
+                    else
+                        for (int q = 0; q < 5; ++q) a.Sout[o + q * F] = cur[q];
+                }
+            }
+        }
+        // rotate the column
+        umz = q0.u;
+        vmz = q0.v;
+        wmz = q0.w;
+        q0 = q1;
+        for (int q = 0; q < 5; ++q) {
+            gz0[q] = gz1[q];
+            phizm[q] = phiz[q];
+        }
+    }
+}
+
+template <class T, int TY>
+void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a0, int kc) {
+    FusedArgs<T> a = a0;
+    a.nbx = (a.L.NI + 61) / 62;
+    a.nby = (a.L.NJ + TY - 1) / TY;
+    a.kc = kc;
+    a.nbz = (a.L.NK + a.kc - 1) / a.kc;
+    a.nblocks = a.nbx * a.nby * a.nbz;
+    const int grid = (a.nblocks + 7) / 8 * 8;
+    dim3 blk(64, TY + 2);
+    switch (stage) {
+        case 1: hipLaunchKernelGGL((k_stage_fused<T, 1, TY>), dim3(grid), blk, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_stage_fused<T, 2, TY>), dim3(grid), blk, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage_fused<T, 3, TY>), dim3(grid), blk, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_stage_fused<T, 4, TY>), dim3(grid), blk, 0, s, a); break;
+    }
+}
+
+}  // namespace
+
+template <class T>
+void launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
+                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
+                      const double* dtdev, FusedTile tile) {
+    FusedArgs<T> a{};
+    a.L = L;
+    a.W = W;
+    const double dx = P.hx, dy = P.hy, dz = P.hz;
+    a.c.i2dx = (T)(1 / (2 * dx));
+    a.c.i2dy = (T)(1 / (2 * dy));
+    a.c.i2dz = (T)(1 / (2 * dz));
+    a.c.c43x = (T)(2 / (3 * dx * dx));
+    a.c.c43y = (T)(2 / (3 * dy * dy));
+    a.c.c43z = (T)(2 / (3 * dz * dz));
+    a.c.c1x = (T)(1 / (2 * dx * dx));
+    a.c.c1y = (T)(1 / (2 * dy * dy));
+    a.c.c1z = (T)(1 / (2 * dz * dz));
+    a.c.gamma = (T)P.gamma;
+    a.c.gm1 = (T)(P.gamma - 1);
+    a.c.H0 = (T)(1 / (P.gamma - 1));
+    a.c.ScP1 = (T)P.ScPlusOne;
+    a.c.Re = (T)P.Re;
+    a.c.pf = (T)(1 / ((P.gamma - 1) * P.Pr));
+    a.Sin = Sin;
+    a.U = U;
+    a.Sout = Sout;
+    a.A = A;
+    a.cls = cls;
+    a.rhoPre = rhoPre;
+    a.rhoEPre = rhoEPre;
+    a.dtdev = dtdev;
+    const int kc = tile.kc > 0 ? tile.kc : 64;
+    switch (tile.ty) {
+        case 4: launchTY<T, 4>(s, stage, a, kc); break;
+        case 6: launchTY<T, 6>(s, stage, a, kc); break;
+        case 14: launchTY<T, 14>(s, stage, a, kc); break;
+        default: launchTY<T, 8>(s, stage, a, kc); break;
+    }
+}
+
+const int* fusedTileHeights(int* n) {
+    static const int v[4] = {4, 6, 8, 14};
+    *n = 4;
+    return v;
+}
+
+template void launchStageFused<double>(hipStream_t, int, const Layout&, const Params&, const Window&, const double*,
+                                       const double*, double*, double*, const uint8_t*, const double*, const double*,
+                                       const double*, FusedTile);
+template void launchStageFused<float>(hipStream_t, int, const Layout&, const Params&, const Window&, const float*,
+                                      const float*, float*, float*, const uint8_t*, const float*, const float*,
+                                      const double*, FusedTile);
+
+}  // namespace nsfsi

@@ -106,6 +106,8 @@ def lib():
     L.ns_totals.argtypes = [H, C.POINTER(C.c_double)]
     L.ns_get_geometry.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.ns_profile.argtypes = [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    L.ns_set_option.argtypes = [H, C.c_char_p, C.c_int64]
+    L.ns_get_tile.argtypes = [H, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.ns_last_error.argtypes = [H]
     L.ns_last_error.restype = C.c_char_p
     L.ns_destroy.argtypes = [H]
@@ -113,7 +115,7 @@ def lib():
     L.ns_abi_version.restype = C.c_int32
     for name in ("ns_config_parse", "ns_create", "ns_init", "ns_set_initial_field", "ns_step", "ns_run", "ns_get_time",
                  "ns_get_field", "ns_get_node_types", "ns_get_counts", "ns_get_ghosts", "ns_get_filter_nodes",
-                 "ns_norm_history", "ns_totals", "ns_get_geometry", "ns_profile"):
+                 "ns_norm_history", "ns_totals", "ns_get_geometry", "ns_profile", "ns_set_option", "ns_get_tile"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -313,6 +315,21 @@ class Solver:
     @property
     def dt(self):
         return self.time()[2]
+
+    def setOption(self, name: str, value: int):
+        """Run-time option, e.g. setOption("stage_kernel", 0) for the unfused stage."""
+        self._check(lib().ns_set_option(self._h, name.encode(), int(value)))
+
+    def tile(self):
+        """(rows per block, z-planes per block) of the fused stage."""
+        ty, kc = C.c_int32(), C.c_int32()
+        self._check(lib().ns_get_tile(self._h, C.byref(ty), C.byref(kc)))
+        return ty.value, kc.value
+
+    def autotune(self, reps: int = 3):
+        """Times every fused-stage tile shape on this case and keeps the fastest; returns it."""
+        self.setOption("autotune", reps)
+        return self.tile()
 
     def profile(self, nsteps: int):
         """Mean ms per launch of (stage kernel, primitive pass, boundary kernels, reductions) and launches/step."""

@@ -46,6 +46,21 @@ def cpu_baseline(workload, precision_unused=None):
     }
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes (a kernel of known traffic for scaling FETCH/WRITE counters)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +70,10 @@ def main():
     ap.add_argument("--precision", default=None)
     ap.add_argument("--N", type=int, default=None, help="override the grid edge")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", default="fused", choices=["fused", "unfused"])
+    ap.add_argument("--tile", default=None, help="fixed fused tile 'rows,zplanes' instead of autotuning")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, one device-to-device copy of MIB MiB (counter calibration under rocprofv3)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build's path runs on a single MI355X (no multi-GPU sharding); "
@@ -74,6 +93,16 @@ def main():
     nodes = cfg.NI * cfg.NJ * cfg.NK
     s = nsfsi.Solver(cfg)
     s.initialize()
+    s.setOption("stage_kernel", 1 if a.kernel == "fused" else 0)
+    if a.kernel != "fused":
+        tile = None
+    elif a.tile:
+        ty, kc = (int(x) for x in a.tile.split(","))
+        s.setOption("tile_ty", ty)
+        s.setOption("tile_kc", kc)
+        tile = s.tile()
+    else:
+        tile = s.autotune(2)
     s.marchTimeStep(a.warmup)
     t0 = time.perf_counter()
     s.marchTimeStep(a.steps)  # returns after a device synchronisation
@@ -101,7 +130,8 @@ def main():
         "dtype": "f32" if prec == "fp32" else "f64",
         "data": "synthetic (seeded freestream + perturbation initial field)",
         "config": {"workload": a.workload, "grid": [cfg.NI, cfg.NJ, cfg.NK], "precision": prec,
-                   "ghosts": g, "filter_nodes": f, "stage_kernel": "plain (primitive pass + per-node RHS)"},
+                   "ghosts": g, "filter_nodes": f, "stage_kernel": a.kernel,
+                   "tile_rows_x_zplanes": list(tile) if tile else None},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1) if achieved else None,
@@ -109,7 +139,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": None,
-            "kernel": "k_rhs_plain (RK4 stage)",
+            "kernel": "k_stage_fused (RK4 stage)" if a.kernel == "fused" else "k_rhs_plain (RK4 stage)",
             "kernel_ms": round(ms[0], 4),
             "step_breakdown_ms": {"stage_kernel": round(ms[0] * launches[0], 3),
                                   "primitive_pass": round(ms[1] * launches[1], 3),
@@ -119,6 +149,8 @@ def main():
         },
     }
     s.close()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration << 20)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.workload)
     print(json.dumps(line), flush=True)

@@ -172,6 +172,18 @@ ns_status ns_get_geometry(ns_handle* h, double spacing[3], double offset[3]);
  * step. The steps advance the solution like ns_step. */
 ns_status ns_profile(ns_handle* h, int32_t nsteps, double ms[4], int32_t launches[4]);
 
+/* Run-time option (no reference counterpart):
+ *   "stage_kernel" 1 (default) fused stage kernel, 0 the unfused pair (primitive pass + per-node RHS);
+ *   "tile_ty"      output rows per block of the fused stage (4, 6, 8 or 14; default 8);
+ *   "tile_kc"      z-planes per block of the fused stage (default 64);
+ *   "autotune"     after ns_init: time every (tile_ty, tile_kc) pair on this case with `value` launches
+ *                  each (the state is not changed) and keep the fastest.
+ * Results do not depend on the tile shape (the per-node arithmetic is the same). */
+ns_status ns_set_option(ns_handle* h, const char* name, int64_t value);
+
+/* Tile shape in use by the fused stage (after autotuning, the chosen one). */
+ns_status ns_get_tile(ns_handle* h, int32_t* ty, int32_t* kc);
+
 /* Last error text of this handle (never null). */
 const char* ns_last_error(ns_handle* h);
 
