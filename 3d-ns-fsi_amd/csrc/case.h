@@ -49,11 +49,17 @@ public:
     // Run-time options: "stage_kernel" 0 = unfused (primitive pass + per-node RHS), 1 = fused.
     void setOption(const std::string& name, int64_t value);
     int stageKernel = 1;
+    bool fusedScans = false;  // end-of-step scans inside the stage-4 kernel (else a separate full pass)
     FusedTile tile{};
     // Times every fused-stage tile shape on this case (stage-2 launches into scratch outputs, state
     // untouched) and keeps the fastest; returns the chosen tile and its time per launch in ms.
     double autotune(int reps);
     std::vector<double> tuneTimes;  // ms per launch of each tried shape, in try order
+    // HIP-graph replay of the time step: one graph per buffer parity (which allocation holds the state)
+    bool useGraph = false;
+    hipGraphExec_t graphExec[2] = {nullptr, nullptr};
+    void* graphState[2] = {nullptr, nullptr};
+    void dropGraphs();
 
     ns_config cfg;
     bool fp32 = false;
@@ -83,6 +89,7 @@ public:
     StepScalars* scal = nullptr;
     double* partials = nullptr;
     int nPartials = 0;
+    int nStage4Partials = 0;
     double* hist = nullptr;
     int histCap = 1024;
     std::vector<double> normsHost[5];

@@ -49,41 +49,43 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
 template <class T>
 __global__ void k_step_reduce(Layout L, Params P, const T* U, const T* A, const uint8_t* cls, Window W, const T* rhoPre,
                               const T* rhoEPre, int withNorms, double* partials) {
+    // one block per group of x-rows: no per-node index division
     __shared__ double sh[8];
     const size_t F = L.fstride;
-    const size_t N = L.nodes();
+    const int rows = L.NJ * L.NK;
     Eos<T> e = Eos<T>::make(P);
     const double vm = fmax(4. / 3, P.gamma / P.Pr);
     double mx[4] = {0, 0, 0, 0}, sm[5] = {0, 0, 0, 0, 0}, bad = 0;
-    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(n % L.NI);
-        size_t r = n / L.NI;
-        int j = (int)(r % L.NJ), k = (int)(r / L.NJ);
-        size_t o = L.off(i, j, k);
-        uint8_t c = cls[o];
-        double a[5];
-        for (int q = 0; q < 5; ++q) a[q] = (double)A[o + q * F];
-        for (int q = 0; q < 5; ++q)
-            if (!isfinite(a[q])) bad += 1;
-        if (withNorms && (c & CLS_TYPE_MASK) == NS_FLUID_INTERIOR)
-            for (int q = 0; q < 5; ++q) {
-                double d = (double)U[o + q * F] - a[q];
-                sm[q] += d * d;
+    for (int rw = blockIdx.x; rw < rows; rw += gridDim.x) {
+        const int j = rw % L.NJ, k = rw / L.NJ;
+        const bool rowW = W.ni > 0 && j >= W.j0 && j < W.j0 + W.nj && k >= W.k0 && k < W.k0 + W.nk;
+        for (int i = threadIdx.x; i < L.NI; i += blockDim.x) {
+            const size_t o = L.off(i, j, k);
+            const uint8_t c = cls[o];
+            T a[5];
+            for (int q = 0; q < 5; ++q) a[q] = A[o + q * F];
+            for (int q = 0; q < 5; ++q)
+                if (!isfinite(a[q])) bad += 1;
+            if (withNorms && (c & CLS_TYPE_MASK) == NS_FLUID_INTERIOR)
+                for (int q = 0; q < 5; ++q) {
+                    double d = (double)U[o + q * F] - (double)a[q];
+                    sm[q] += d * d;
+                }
+            if (c & (CLS_DT_SKIP | CLS_INACTIVE_IC)) continue;
+            T rho = a[0], rE = a[4];
+            if (rowW && (c & CLS_FILTER)) {
+                size_t w = W.idx(i, j, k);
+                rho = rhoPre[w];
+                rE = rhoEPre[w];
             }
-        if (c & (CLS_DT_SKIP | CLS_INACTIVE_IC)) continue;
-        T rho = A[o], rE = A[o + 4 * F];
-        if (c & CLS_FILTER) {
-            size_t w = W.idx(i, j, k);
-            rho = rhoPre[w];
-            rE = rhoEPre[w];
+            T u, v, ww, p, Tt, mu;
+            derivePrims(e, rho, a[1], a[2], a[3], rE, u, v, ww, p, Tt, mu);
+            double cs = sqrt((1 + P.gamma * (double)p) / (1 + (double)a[0]));
+            mx[0] = fmax(mx[0], cs + fabs((double)u));
+            mx[1] = fmax(mx[1], cs + fabs((double)v));
+            mx[2] = fmax(mx[2], cs + fabs((double)ww));
+            mx[3] = fmax(mx[3], (double)mu / ((double)a[0] + 1) * vm);
         }
-        T u, v, w, p, Tt, mu;
-        derivePrims(e, rho, A[o + F], A[o + 2 * F], A[o + 3 * F], rE, u, v, w, p, Tt, mu);
-        double cs = sqrt((1 + P.gamma * (double)p) / (1 + a[0]));
-        mx[0] = fmax(mx[0], cs + fabs((double)u));
-        mx[1] = fmax(mx[1], cs + fabs((double)v));
-        mx[2] = fmax(mx[2], cs + fabs((double)w));
-        mx[3] = fmax(mx[3], (double)mu / (a[0] + 1) * vm);
     }
     double* out = partials + (size_t)blockIdx.x * NP;
     for (int q = 0; q < 4; ++q) {
@@ -166,7 +168,115 @@ __global__ void k_finalize(Params P, const double* partials, int nPartials, int 
     st->dt = dt;
 }
 
+struct FaceSet {
+    EdgeFace f[6];
+    int64_t start[7];  // prefix sums of the owned-node counts
+};
+
+// Owned face nodes, ghosts and filter nodes after the final boundary application of a step.
+template <class T>
+__global__ void k_boundary_reduce(Layout L, Params P, Window W, FaceSet fs, const int64_t* ghostNode, int64_t nGhost,
+                                  const int64_t* filterList, int64_t nFilter, const T* U, const T* A, const uint8_t* cls,
+                                  const T* rhoPre, const T* rhoEPre, int norms, double* partials) {
+    __shared__ double sh[8];
+    const size_t F = L.fstride;
+    Eos<T> e = Eos<T>::make(P);
+    const double vm = fmax(4. / 3, P.gamma / P.Pr);
+    const int64_t nFace = fs.start[6];
+    const int64_t total = nFace + nGhost + nFilter;
+    double mx[4] = {0, 0, 0, 0}, sm[5] = {0, 0, 0, 0, 0}, bad = 0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        size_t o;
+        bool filt = false;
+        size_t w = 0;
+        if (q < nFace) {
+            int f = 0;
+            while (q >= fs.start[f + 1]) ++f;
+            const EdgeFace& ef = fs.f[f];
+            int64_t r = q - fs.start[f];
+            int lo[3] = {ef.i0, ef.j0, ef.k0}, hi[3] = {ef.i1, ef.j1, ef.k1};
+            int ax1 = ef.axis == 0 ? 1 : 0, ax2 = ef.axis == 2 ? 1 : 2;
+            int n1 = hi[ax1] - lo[ax1] + 1;
+            int c3[3];
+            c3[ef.axis] = lo[ef.axis];
+            c3[ax1] = lo[ax1] + (int)(r % n1);
+            c3[ax2] = lo[ax2] + (int)(r / n1);
+            o = L.off(c3[0], c3[1], c3[2]);
+            if (cls[o] & CLS_DT_SKIP) continue;
+        } else if (q < nFace + nGhost) {
+            o = (size_t)ghostNode[q - nFace];
+        } else {
+            int64_t m = filterList[q - nFace - nGhost];
+            int i = (int)(m % L.NI);
+            int64_t r = m / L.NI;
+            int j = (int)(r % L.NJ), k = (int)(r / L.NJ);
+            o = L.off(i, j, k);
+            filt = true;
+            w = W.idx(i, j, k);
+        }
+        double a[5];
+        for (int c = 0; c < 5; ++c) a[c] = (double)A[o + c * F];
+        for (int c = 0; c < 5; ++c)
+            if (!isfinite(a[c])) bad += 1;
+        T rho = A[o], rE = A[o + 4 * F];
+        if (filt) {
+            if (norms)
+                for (int c = 0; c < 5; ++c) {
+                    double d = (double)U[o + c * F] - a[c];
+                    sm[c] += d * d;
+                }
+            rho = rhoPre[w];
+            rE = rhoEPre[w];
+        }
+        T u, v, ww, p, Tt, mu;
+        derivePrims(e, rho, A[o + F], A[o + 2 * F], A[o + 3 * F], rE, u, v, ww, p, Tt, mu);
+        double cs = sqrt((1 + P.gamma * (double)p) / (1 + a[0]));
+        mx[0] = fmax(mx[0], cs + fabs((double)u));
+        mx[1] = fmax(mx[1], cs + fabs((double)v));
+        mx[2] = fmax(mx[2], cs + fabs((double)ww));
+        mx[3] = fmax(mx[3], (double)mu / (a[0] + 1) * vm);
+    }
+    double* out = partials + (size_t)blockIdx.x * NP;
+    for (int q = 0; q < 4; ++q) {
+        double v = blockMax(mx[q], sh);
+        if (threadIdx.x == 0) out[q] = v;
+    }
+    for (int q = 0; q < 5; ++q) {
+        double v = blockSum(sm[q], sh);
+        if (threadIdx.x == 0) out[4 + q] = v;
+    }
+    double b = blockSum(bad, sh);
+    if (threadIdx.x == 0) out[9] = b;
+}
+
 }  // namespace
+
+template <class T>
+int launchBoundaryReduce(hipStream_t s, const Layout& L, const Params& P, const Window& W, const EdgeFace faces[6],
+                         const int64_t* ghostNode, int64_t nGhost, const int64_t* filterList, int64_t nFilter,
+                         const T* U, const T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre, int norms,
+                         double* partials) {
+    FaceSet fs;
+    fs.start[0] = 0;
+    for (int f = 0; f < 6; ++f) {
+        fs.f[f] = faces[f];
+        const EdgeFace& e = faces[f];
+        int64_t n = (int64_t)(e.i1 - e.i0 + 1) * (e.j1 - e.j0 + 1) * (e.k1 - e.k0 + 1);
+        fs.start[f + 1] = fs.start[f] + std::max<int64_t>(n, 0);
+    }
+    int64_t total = fs.start[6] + nGhost + nFilter;
+    int nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, (total + RB - 1) / RB));
+    hipLaunchKernelGGL(k_boundary_reduce<T>, dim3(nb), dim3(RB), 0, s, L, P, W, fs, ghostNode, nGhost, filterList, nFilter,
+                       U, A, cls, rhoPre, rhoEPre, norms, partials);
+    return nb;
+}
+
+template int launchBoundaryReduce<double>(hipStream_t, const Layout&, const Params&, const Window&, const EdgeFace[6],
+                                          const int64_t*, int64_t, const int64_t*, int64_t, const double*, const double*,
+                                          const uint8_t*, const double*, const double*, int, double*);
+template int launchBoundaryReduce<float>(hipStream_t, const Layout&, const Params&, const Window&, const EdgeFace[6],
+                                         const int64_t*, int64_t, const int64_t*, int64_t, const float*, const float*,
+                                         const uint8_t*, const float*, const float*, int, double*);
 
 template <class T>
 void launchStepReduce(hipStream_t s, const Layout& L, const Params& P, const T* U, const T* A, const uint8_t* cls,

@@ -153,6 +153,7 @@ Case::Case(const ns_config& c) : cfg(c) {
 }
 
 Case::~Case() {
+    dropGraphs();
     for (void* p : allocations) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -314,7 +315,8 @@ void Case::allocFields() {
     rhoEPre = alloc(wn * sizeof(T));
     ftmp = alloc(2 * std::max<int64_t>(1, nFilter) * sizeof(T));
     nPartials = 1024;
-    partials = (double*)alloc((size_t)(nPartials + 64) * reducePartialSlots() * sizeof(double));
+    size_t slots = (size_t)std::max(nPartials, fusedMaxBlocks(L)) + 512 + 64;
+    partials = (double*)alloc(slots * reducePartialSlots() * sizeof(double));
     scal = (StepScalars*)alloc(sizeof(StepScalars));
     hist = (double*)alloc((size_t)histCap * 5 * sizeof(double));
     StepScalars s0{};
@@ -435,8 +437,9 @@ void Case::stepOnce() {
     for (int st = 1; st <= 4; ++st) {
         if (stageKernel == 1) {
             mark(0, true);
-            launchStageFused<T>(stream, st, L, P, W, in[st - 1], u, out[st - 1], a, cls, (const T*)rhoPre,
-                                (const T*)rhoEPre, dtdev, tile);
+            nStage4Partials = launchStageFused<T>(stream, st, L, P, W, in[st - 1], u, out[st - 1], a, cls,
+                                                  (const T*)rhoPre, (const T*)rhoEPre, dtdev, tile,
+                                                  fusedScans ? partials : nullptr, cfg.conv_history > 0);
             mark(0, false);
         } else {
             mark(1, true);
@@ -451,15 +454,37 @@ void Case::stepOnce() {
         mark(2, false);
     }
     mark(3, true);
-    computeDt<T>(1);
+    if (stageKernel == 1 && fusedScans) {
+        // scans of the interior were done by the stage-4 kernel; add the faces, ghosts, filter nodes and
+        // untouched solid nodes, then advance t / timeLevel / dt
+        double* pb = partials + (size_t)nStage4Partials * reducePartialSlots();
+        int nb = launchBoundaryReduce<T>(stream, L, P, W, faces, G.node, nGhost, filterList, nFilter, (const T*)U,
+                                         (const T*)A, cls, (const T*)rhoPre, (const T*)rhoEPre, cfg.conv_history > 0,
+                                         pb);
+        int npart = nStage4Partials + nb;
+        if (nInactive > 0) {
+            launchInactiveReduce<T>(stream, L, P, (const T*)A, inactiveList, inactiveIC, nInactive,
+                                    partials + (size_t)npart * reducePartialSlots());
+            npart += inactiveReduceBlocks(nInactive);
+        }
+        launchFinalize(stream, P, partials, npart, 1, cfg.conv_history, cfg.stop_criterion == 1, cfg.t_end,
+                       cfg.conv_stability_limit, cfg.visc_stability_limit, scal, hist, histCap);
+    } else {
+        computeDt<T>(1);
+    }
     mark(3, false);
     std::swap(U, A);
 }
 
 void Case::setOption(const std::string& name, int64_t value) {
+    dropGraphs();
     if (name == "stage_kernel") {
         if (value != 0 && value != 1) throw std::invalid_argument("stage_kernel must be 0 or 1");
         stageKernel = (int)value;
+    } else if (name == "fused_scans") {
+        fusedScans = value != 0;
+    } else if (name == "graph") {
+        useGraph = value != 0;
     } else if (name == "tile_ty") {
         int n = 0;
         const int* v = fusedTileHeights(&n);
@@ -499,11 +524,12 @@ double Case::autotune(int reps) {
                 for (int r = 0; r < count; ++r) {
                     if (fp32)
                         launchStageFused<float>(stream, 2, L, P, W, (const float*)U, (const float*)U, (float*)S2, (float*)A,
-                                                cls, (const float*)rhoPre, (const float*)rhoEPre, &scal->dt, tl);
+                                                cls, (const float*)rhoPre, (const float*)rhoEPre, &scal->dt, tl,
+                                                nullptr, 0);
                     else
                         launchStageFused<double>(stream, 2, L, P, W, (const double*)U, (const double*)U, (double*)S2,
                                                  (double*)A, cls, (const double*)rhoPre, (const double*)rhoEPre,
-                                                 &scal->dt, tl);
+                                                 &scal->dt, tl, nullptr, 0);
                 }
             };
             run(1);  // warm-up
@@ -617,6 +643,14 @@ void Case::readScalars() {
     }
 }
 
+void Case::dropGraphs() {
+    for (int q = 0; q < 2; ++q) {
+        if (graphExec[q]) (void)hipGraphExecDestroy(graphExec[q]);
+        graphExec[q] = nullptr;
+        graphState[q] = nullptr;
+    }
+}
+
 void Case::step(int nsteps) {
     if (!initialized) throw StateError("step before init");
     if (!filterDisabled()) throw Unsupported("the periodic density filter (filterIntervals) is not implemented on the device");
@@ -624,8 +658,30 @@ void Case::step(int nsteps) {
     while (done < nsteps) {
         int batch = std::min(nsteps - done, histCap);
         for (int q = 0; q < batch; ++q) {
-            if (fp32) stepOnce<float>();
-            else stepOnce<double>();
+            if (useGraph) {
+                // the step reads dt and t from device memory and never synchronises, so one captured
+                // step replays as is; the U/A swap alternates the buffers, hence one graph per parity
+                int slot = graphState[0] == U ? 0 : (graphState[1] == U ? 1 : -1);
+                if (slot < 0) {
+                    slot = graphExec[0] ? 1 : 0;
+                    void* start = U;
+                    hipGraph_t g;
+                    hipCheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+                    if (fp32) stepOnce<float>();
+                    else stepOnce<double>();
+                    hipCheck(hipStreamEndCapture(stream, &g), "end capture");
+                    std::swap(U, A);  // capture recorded the launches only; undo the host-side swap
+                    hipCheck(hipGraphInstantiate(&graphExec[slot], g, nullptr, nullptr, 0), "instantiate");
+                    (void)hipGraphDestroy(g);
+                    graphState[slot] = start;
+                }
+                hipCheck(hipGraphLaunch(graphExec[slot], stream), "graph launch");
+                std::swap(U, A);
+            } else if (fp32) {
+                stepOnce<float>();
+            } else {
+                stepOnce<double>();
+            }
         }
         hipCheck(hipGetLastError(), "step kernels");
         readScalars();

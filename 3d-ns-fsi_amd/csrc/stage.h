@@ -37,11 +37,23 @@ struct FusedTile {
     int ty = 8;
     int kc = 64;
 };
+// Stage 4 with partials != null also writes one partial per block of the end-of-step scans over
+// interior, non-filter nodes (norms of change when norms != 0, dt maxima, non-finite count). Returns the
+// number of blocks (= partial slots written).
 template <class T>
-void launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
-                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
-                      const double* dtdev, FusedTile tile);
+int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
+                     const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
+                     const double* dtdev, FusedTile tile, double* partials, int norms);
 const int* fusedTileHeights(int* n);
+int fusedMaxBlocks(const Layout& L);  // largest block count over all tile shapes
+
+// End-of-step scans over what the stage-4 pass leaves out, after the boundary conditions: the owned
+// nodes of the six faces, the ghost nodes and the filter nodes. Returns the partial slots written.
+template <class T>
+int launchBoundaryReduce(hipStream_t s, const Layout& L, const Params& P, const Window& W, const EdgeFace faces[6],
+                         const int64_t* ghostNode, int64_t nGhost, const int64_t* filterList, int64_t nFilter,
+                         const T* U, const T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre, int norms,
+                         double* partials);
 
 // ---- end-of-step reductions (reduce.hip)
 template <class T>

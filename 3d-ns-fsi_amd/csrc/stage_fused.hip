@@ -28,6 +28,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <stdexcept>
+
 #include "device_common.h"
 #include "stage.h"
 
@@ -48,22 +51,54 @@ struct FusedArgs {
     Layout L;
     Window W;
     Coef<T> c;
-    const T* Sin;
-    const T* U;
-    T* Sout;
-    T* A;
+    // one base pointer per conserved field: loads and stores then share one 32-bit byte offset per
+    // node (scalar base + vector offset addressing, no per-access 64-bit address arithmetic)
+    const T* Sin[5];
+    const T* U[5];
+    T* Sout[5];
+    T* A[5];
     const uint8_t* cls;
     const T* rhoPre;
     const T* rhoEPre;
     const double* dtdev;
     int nbx, nby, nbz, kc;   // tiles in x, y, z and planes per z-chunk
     int nblocks;
+    // stage 4 only: per-block partials of the end-of-step scans over interior, non-filter nodes
+    // (slots: 3 spectral-radius maxima, viscous maximum, 5 sums of squared change, non-finite count)
+    double* part;
+    int norms;
+    T vm;  // max(4/3, gamma/Pr)
 };
 
+// Whole-wavefront shifts by one lane (DPP wave_shl:1 / wave_shr:1 of GFX9): shl1 gives each lane the
+// value of lane + 1, shr1 the value of lane - 1 (the edge lane receives 0; edge lanes are halo lanes).
+__device__ __forceinline__ float shl1(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float shr1(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double shl1(double x) {
+    int2 v = *reinterpret_cast<int2*>(&x);
+    v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x130, 0xf, 0xf, false);
+    v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x130, 0xf, 0xf, false);
+    return *reinterpret_cast<double*>(&v);
+}
+__device__ __forceinline__ double shr1(double x) {
+    int2 v = *reinterpret_cast<int2*>(&x);
+    v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x138, 0xf, 0xf, false);
+    v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x138, 0xf, 0xf, false);
+    return *reinterpret_cast<double*>(&v);
+}
+
 template <class T>
-__device__ __forceinline__ T shl1(T x) { return __shfl_down(x, 1, 64); }  // value of lane + 1
+__device__ __forceinline__ T ldAt(const T* base, uint32_t boff) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
 template <class T>
-__device__ __forceinline__ T shr1(T x) { return __shfl_up(x, 1, 64); }    // value of lane - 1
+__device__ __forceinline__ void stAt(T* base, uint32_t boff, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
 
 // Point state at one plane of the own column.
 template <class T>
@@ -74,17 +109,24 @@ struct Pt {
     T dxu, dxv, dxw, dyu, dyv, dyw;  // in-plane central derivatives
 };
 
+// Reciprocal and square root: IEEE in FP64 (the parity form); in FP32 the hardware's 1-ulp
+// v_rcp_f32 / v_sqrt_f32, which replace multi-instruction correctly-rounded sequences.
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double sqrtT(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrtT(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 template <class T>
 __device__ __forceinline__ void prims(const Coef<T>& c, T rho, T ru, T rv, T rw, T E, Pt<T>& q) {
     T m = (T)1 + rho;
-    T im = (T)1 / m;
+    T im = rcp(m);
     q.u = ru * im;
     q.v = rv * im;
     q.w = rw * im;
     q.p = c.gm1 * (E - m * (T)0.5 * (q.u * q.u + q.v * q.v + q.w * q.w));
     q.Tt = (c.gamma * q.p - rho) * im;
     T a = (T)1 + q.Tt;
-    q.mu = a * sqrt(a) * c.ScP1 / (c.Re * (q.Tt + c.ScP1));
+    q.mu = a * sqrtT(a) * c.ScP1 * rcp(c.Re * (q.Tt + c.ScP1));
 }
 
 // Compact face terms between a and b along one direction; k43 is the coefficient of the velocity
@@ -155,13 +197,13 @@ __device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, int i, int j, int
     const Layout& L = a.L;
     const int ic = min(max(i, 0), L.NI - 1), jc = min(max(j, 0), L.NJ - 1), kc = min(max(k, 0), L.NK - 1);
     const size_t o = L.off(ic, jc, kc);
-    const size_t F = L.fstride;
+    const uint32_t bo = (uint32_t)(o * sizeof(T));
     Raw<T> r;
-    r.rho = a.Sin[o];
-    r.ru = a.Sin[o + F];
-    r.rv = a.Sin[o + 2 * F];
-    r.rw = a.Sin[o + 3 * F];
-    r.E = a.Sin[o + 4 * F];
+    r.rho = ldAt(a.Sin[0], bo);
+    r.ru = ldAt(a.Sin[1], bo);
+    r.rv = ldAt(a.Sin[2], bo);
+    r.rw = ldAt(a.Sin[3], bo);
+    r.E = ldAt(a.Sin[4], bo);
     r.rp = r.rho;
     r.Ep = r.E;
     if (a.W.contains(ic, jc, kc) && (a.cls[o] & CLS_FILTER)) {
@@ -175,7 +217,7 @@ __device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, int i, int j, int
 // Primitives and in-plane derivatives of a fetched plane; x-derivatives by shuffle, y through LDS.
 template <class T, int TY>
 __device__ __forceinline__ void finishPlane(const FusedArgs<T>& a, Smem<T, TY>& sm, int row, int lane, const Raw<T>& r,
-                                            Pt<T>& q) {
+                                            Pt<T>& q, bool needDy) {
     q.rho = r.rho;
     q.ru = r.ru;
     q.rv = r.rv;
@@ -190,10 +232,14 @@ __device__ __forceinline__ void finishPlane(const FusedArgs<T>& a, Smem<T, TY>& 
     sm.uvw[1][row][lane] = q.v;
     sm.uvw[2][row][lane] = q.w;
     __syncthreads();
-    const int rn = min(row + 1, TY + 1), rs = max(row - 1, 0);
-    q.dyu = (sm.uvw[0][rn][lane] - sm.uvw[0][rs][lane]) * a.c.i2dy;
-    q.dyv = (sm.uvw[1][rn][lane] - sm.uvw[1][rs][lane]) * a.c.i2dy;
-    q.dyw = (sm.uvw[2][rn][lane] - sm.uvw[2][rs][lane]) * a.c.i2dy;
+    // halo rows never use y-derivatives (they only supply G_y, which needs x- and z-derivatives)
+    if (needDy) {
+        q.dyu = (sm.uvw[0][row + 1][lane] - sm.uvw[0][row - 1][lane]) * a.c.i2dy;
+        q.dyv = (sm.uvw[1][row + 1][lane] - sm.uvw[1][row - 1][lane]) * a.c.i2dy;
+        q.dyw = (sm.uvw[2][row + 1][lane] - sm.uvw[2][row - 1][lane]) * a.c.i2dy;
+    } else {
+        q.dyu = q.dyv = q.dyw = 0;
+    }
 }
 
 template <class T, int TY>
@@ -206,8 +252,10 @@ __device__ __forceinline__ void readFaceNb(const Smem<T, TY>& sm, int r, int lan
     nb.Tt = sm.face[9][r][lane];
 }
 
-template <class T, int STAGE, int TY>
-__global__ __launch_bounds__(64 * (TY + 2)) void k_stage_fused(FusedArgs<T> a) {
+// FP32: at least 4 waves per SIMD (<= 128 VGPRs); FP64: at least 2.
+template <class T, int STAGE, int TY, bool SCAN>
+__global__ __launch_bounds__(64 * (TY + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+void k_stage_fused(FusedArgs<T> a) {
     __shared__ Smem<T, TY> sm;
     // XCD-aware mapping: hardware block p runs on XCD p % 8; give each XCD a contiguous range of tiles
     const int p = blockIdx.x;
@@ -219,7 +267,6 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_stage_fused(FusedArgs<T> a) {
     const int bz = lb / (a.nbx * a.nby);
     const int lane = threadIdx.x, row = threadIdx.y;
     const Layout& L = a.L;
-    const size_t F = L.fstride;
     const int i = bx * 62 + lane - 1;
     const int j = by * TY + row - 1;
     const int kb = bz * a.kc, ke = min(L.NK, kb + a.kc);
@@ -231,20 +278,27 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_stage_fused(FusedArgs<T> a) {
     // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
     Pt<T> qm, q0, q1;
     Raw<T> rnext = fetch(a, i, j, kb - 1);
-    finishPlane<T, TY>(a, sm, row, lane, rnext, qm);
+    const bool inner = row >= 1 && row <= TY;  // output rows (halo rows are 0 and TY+1)
+    finishPlane<T, TY>(a, sm, row, lane, rnext, qm, inner);
     rnext = fetch(a, i, j, kb);
     __syncthreads();
-    finishPlane<T, TY>(a, sm, row, lane, rnext, q0);
+    finishPlane<T, TY>(a, sm, row, lane, rnext, q0, inner);
     rnext = fetch(a, i, j, kb + 1);
     T gz0[5], phizm[5];
-    {
+    if (inner) {
         T gzm[5], cz[5];
         gz(c, qm, gzm);
         gz(c, q0, gz0);
         compact<T, 2>(c, qm, q0, cz);
         for (int q = 0; q < 5; ++q) phizm[q] = (gzm[q] + gz0[q]) * c.i2dz + cz[q];
+    } else {
+        for (int q = 0; q < 5; ++q) gz0[q] = phizm[q] = 0;
     }
     T umz = qm.u, vmz = qm.v, wmz = qm.w;
+    // per-thread scan accumulators in the field type (a thread sees at most one z-chunk of nodes); the
+    // block partials are combined in FP64
+    T red[10];
+    for (int q = 0; q < 10; ++q) red[q] = 0;
     __syncthreads();  // the prologue's readers of sm.uvw are done before the loop rewrites it
 
     for (int k = kb; k < ke; ++k) {
@@ -253,31 +307,39 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_stage_fused(FusedArgs<T> a) {
         // combine inputs of the output node, fetched before the stencil work
         const bool doOut = outRow && outLane;
         const size_t o = doOut ? L.off(i, j, k) : 0;
-        bool interior = false;
+        const uint32_t bo = (uint32_t)(o * sizeof(T));
+        bool interior = false, filterNode = false;
         T base[5], acc[5];
+        (void)filterNode;
         if (doOut) {
             interior = i >= 1 && i <= L.NI - 2 && j >= 1 && j <= L.NJ - 2 && k >= 1 && k <= L.NK - 2;
-            if (interior && a.W.contains(i, j, k)) interior = (a.cls[o] & CLS_TYPE_MASK) == NS_FLUID_INTERIOR;
-            if (STAGE == 2 || STAGE == 3) {
+            if (interior && a.W.contains(i, j, k)) {
+                const uint8_t cl = a.cls[o];
+                interior = (cl & CLS_TYPE_MASK) == NS_FLUID_INTERIOR;
+                filterNode = (cl & CLS_FILTER) != 0;
+            }
+            if (STAGE == 2 || STAGE == 3 || (SCAN && a.norms)) {
                 if (interior)
-                    for (int q = 0; q < 5; ++q) base[q] = a.U[o + q * F];
+                    for (int q = 0; q < 5; ++q) base[q] = ldAt(a.U[q], bo);
             }
             if (STAGE >= 2) {
                 if (interior)
-                    for (int q = 0; q < 5; ++q) acc[q] = a.A[o + q * F];
+                    for (int q = 0; q < 5; ++q) acc[q] = ldAt((const T*)a.A[q], bo);
             } else if (!interior) {
-                for (int q = 0; q < 5; ++q) acc[q] = a.A[o + q * F];
+                for (int q = 0; q < 5; ++q) acc[q] = ldAt((const T*)a.A[q], bo);
             }
         }
         // sm.uvw is rewritten here: every reader of the previous plane's uvw has passed the face barrier
-        finishPlane<T, TY>(a, sm, row, lane, rcur, q1);
-        // z-face above plane k
+        finishPlane<T, TY>(a, sm, row, lane, rcur, q1, inner);
+        // z-face above plane k (output rows only)
         T gz1[5], phiz[5];
-        gz(c, q1, gz1);
-        {
+        if (inner) {
+            gz(c, q1, gz1);
             T cz[5];
             compact<T, 2>(c, q0, q1, cz);
             for (int q = 0; q < 5; ++q) phiz[q] = (gz0[q] + gz1[q]) * c.i2dz + cz[q];
+        } else {
+            for (int q = 0; q < 5; ++q) gz1[q] = phiz[q] = 0;
         }
         const T dzu = (q1.u - umz) * c.i2dz, dzv = (q1.v - vmz) * c.i2dz, dzw = (q1.w - wmz) * c.i2dz;
         // y-faces: every row publishes G_y and its compact inputs; output rows evaluate both faces
@@ -324,26 +386,48 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_stage_fused(FusedArgs<T> a) {
                     if (STAGE == 1) {
                         const T h2 = (T)(dtd / 2), h6 = (T)(dtd / 6);
                         for (int q = 0; q < 5; ++q) {
-                            a.Sout[o + q * F] = cur[q] + h2 * rhs[q];
-                            a.A[o + q * F] = cur[q] + h6 * rhs[q];
+                            stAt(a.Sout[q], bo, cur[q] + h2 * rhs[q]);
+                            stAt(a.A[q], bo, cur[q] + h6 * rhs[q]);
                         }
                     } else if (STAGE == 2 || STAGE == 3) {
                         const T hs = (T)(STAGE == 2 ? dtd / 2 : dtd), h3 = (T)(dtd / 3);
                         for (int q = 0; q < 5; ++q) {
-                            a.Sout[o + q * F] = base[q] + hs * rhs[q];
-                            a.A[o + q * F] = acc[q] + h3 * rhs[q];
+                            stAt(a.Sout[q], bo, base[q] + hs * rhs[q]);
+                            stAt(a.A[q], bo, acc[q] + h3 * rhs[q]);
                         }
                     } else {
                         const T h6 = (T)(dtd / 6);
-                        for (int q = 0; q < 5; ++q) a.A[o + q * F] = acc[q] + h6 * rhs[q];
+                        T an[5];
+                        for (int q = 0; q < 5; ++q) {
+                            an[q] = acc[q] + h6 * rhs[q];
+                            stAt(a.A[q], bo, an[q]);
+                        }
+                        // end-of-step scans; filter nodes change once more in the boundary pass and are
+                        // scanned there (src/Solver.cpp:91-126, src/Mesh.cpp:171-178, src/Mesh.h:48-61)
+                        if (SCAN && !filterNode) {
+                            if (a.norms)
+                                for (int q = 0; q < 5; ++q) {
+                                    T d = base[q] - an[q];
+                                    red[4 + q] += d * d;
+                                }
+                            Pt<T> pn;
+                            prims(c, an[0], an[1], an[2], an[3], an[4], pn);
+                            T cs = sqrt(((T)1 + c.gamma * pn.p) / ((T)1 + an[0]));
+                            red[0] = fmax(red[0], cs + fabs(pn.u));
+                            red[1] = fmax(red[1], cs + fabs(pn.v));
+                            red[2] = fmax(red[2], cs + fabs(pn.w));
+                            red[3] = fmax(red[3], pn.mu / (an[0] + (T)1) * a.vm);
+                            for (int q = 0; q < 5; ++q)
+                                if (!isfinite(an[q])) red[9] += 1;
+                        }
                     }
                 } else {
                     if (STAGE == 1)
-                        for (int q = 0; q < 5; ++q) a.Sout[o + q * F] = acc[q];
+                        for (int q = 0; q < 5; ++q) stAt(a.Sout[q], bo, acc[q]);
                     else if (STAGE == 4)
-                        for (int q = 0; q < 5; ++q) a.A[o + q * F] = cur[q];
+                        for (int q = 0; q < 5; ++q) stAt(a.A[q], bo, cur[q]);
                     else
-                        for (int q = 0; q < 5; ++q) a.Sout[o + q * F] = cur[q];
+                        for (int q = 0; q < 5; ++q) stAt(a.Sout[q], bo, cur[q]);
                 }
             }
         }
@@ -357,6 +441,27 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_stage_fused(FusedArgs<T> a) {
             phizm[q] = phiz[q];
         }
     }
+    if (SCAN) {
+        // block partials: wave reduction, then one value per wave through LDS
+        __shared__ double wred[TY + 2][10];
+        for (int q = 0; q < 10; ++q) {
+            double v = (double)red[q];
+            if (q < 4) v = wave_max(v);
+            else v = wave_sum(v);
+            if (lane == 0) wred[row][q] = v;
+        }
+        __syncthreads();
+        if (row == 0 && lane < 10) {
+            double v = 0;
+            for (int r = 0; r < TY + 2; ++r) v = lane < 4 ? fmax(v, wred[r][lane]) : v + wred[r][lane];
+            a.part[(size_t)lb * 10 + lane] = v;
+        }
+    }
+}
+
+template <class T, int TY>
+int fusedBlocks(const Layout& L, int kc) {
+    return ((L.NI + 61) / 62) * ((L.NJ + TY - 1) / TY) * ((L.NK + kc - 1) / kc);
 }
 
 template <class T, int TY>
@@ -370,20 +475,26 @@ void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a0, int kc) {
     const int grid = (a.nblocks + 7) / 8 * 8;
     dim3 blk(64, TY + 2);
     switch (stage) {
-        case 1: hipLaunchKernelGGL((k_stage_fused<T, 1, TY>), dim3(grid), blk, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_stage_fused<T, 2, TY>), dim3(grid), blk, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage_fused<T, 3, TY>), dim3(grid), blk, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_stage_fused<T, 4, TY>), dim3(grid), blk, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_stage_fused<T, 1, TY, false>), dim3(grid), blk, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_stage_fused<T, 2, TY, false>), dim3(grid), blk, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage_fused<T, 3, TY, false>), dim3(grid), blk, 0, s, a); break;
+        default:
+            if (a.part) hipLaunchKernelGGL((k_stage_fused<T, 4, TY, true>), dim3(grid), blk, 0, s, a);
+            else hipLaunchKernelGGL((k_stage_fused<T, 4, TY, false>), dim3(grid), blk, 0, s, a);
+            break;
     }
 }
 
 }  // namespace
 
 template <class T>
-void launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
-                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
-                      const double* dtdev, FusedTile tile) {
+int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
+                     const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
+                     const double* dtdev, FusedTile tile, double* partials, int norms) {
     FusedArgs<T> a{};
+    a.part = stage == 4 ? partials : nullptr;
+    a.norms = norms;
+    a.vm = (T)std::max(4. / 3, P.gamma / P.Pr);
     a.L = L;
     a.W = W;
     const double dx = P.hx, dy = P.hy, dz = P.hz;
@@ -402,22 +513,27 @@ void launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P
     a.c.ScP1 = (T)P.ScPlusOne;
     a.c.Re = (T)P.Re;
     a.c.pf = (T)(1 / ((P.gamma - 1) * P.Pr));
-    a.Sin = Sin;
-    a.U = U;
-    a.Sout = Sout;
-    a.A = A;
+    if (L.fstride * sizeof(T) > 0xFFFFFFFFull) throw std::length_error("field larger than 4 GiB: 32-bit offsets");
+    for (int q = 0; q < 5; ++q) {
+        a.Sin[q] = Sin + q * L.fstride;
+        a.U[q] = U + q * L.fstride;
+        a.Sout[q] = Sout ? Sout + q * L.fstride : nullptr;
+        a.A[q] = A + q * L.fstride;
+    }
     a.cls = cls;
     a.rhoPre = rhoPre;
     a.rhoEPre = rhoEPre;
     a.dtdev = dtdev;
     const int kc = tile.kc > 0 ? tile.kc : 64;
     switch (tile.ty) {
-        case 4: launchTY<T, 4>(s, stage, a, kc); break;
-        case 6: launchTY<T, 6>(s, stage, a, kc); break;
-        case 14: launchTY<T, 14>(s, stage, a, kc); break;
-        default: launchTY<T, 8>(s, stage, a, kc); break;
+        case 4: launchTY<T, 4>(s, stage, a, kc); return fusedBlocks<T, 4>(L, kc);
+        case 6: launchTY<T, 6>(s, stage, a, kc); return fusedBlocks<T, 6>(L, kc);
+        case 14: launchTY<T, 14>(s, stage, a, kc); return fusedBlocks<T, 14>(L, kc);
+        default: launchTY<T, 8>(s, stage, a, kc); return fusedBlocks<T, 8>(L, kc);
     }
 }
+
+int fusedMaxBlocks(const Layout& L) { return fusedBlocks<float, 4>(L, 16); }
 
 const int* fusedTileHeights(int* n) {
     static const int v[4] = {4, 6, 8, 14};
@@ -425,11 +541,11 @@ const int* fusedTileHeights(int* n) {
     return v;
 }
 
-template void launchStageFused<double>(hipStream_t, int, const Layout&, const Params&, const Window&, const double*,
-                                       const double*, double*, double*, const uint8_t*, const double*, const double*,
-                                       const double*, FusedTile);
-template void launchStageFused<float>(hipStream_t, int, const Layout&, const Params&, const Window&, const float*,
-                                      const float*, float*, float*, const uint8_t*, const float*, const float*,
-                                      const double*, FusedTile);
+template int launchStageFused<double>(hipStream_t, int, const Layout&, const Params&, const Window&, const double*,
+                                      const double*, double*, double*, const uint8_t*, const double*, const double*,
+                                      const double*, FusedTile, double*, int);
+template int launchStageFused<float>(hipStream_t, int, const Layout&, const Params&, const Window&, const float*,
+                                     const float*, float*, float*, const uint8_t*, const float*, const float*,
+                                     const double*, FusedTile, double*, int);
 
 }  // namespace nsfsi

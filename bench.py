@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--N", type=int, default=None, help="override the grid edge")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="fused", choices=["fused", "unfused"])
+    ap.add_argument("--fused-scans", action="store_true", help="end-of-step scans inside the last stage kernel")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernel by kernel instead of replaying a HIP graph")
     ap.add_argument("--tile", default=None, help="fixed fused tile 'rows,zplanes' instead of autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB (counter calibration under rocprofv3)")
@@ -103,6 +105,8 @@ def main():
         tile = s.tile()
     else:
         tile = s.autotune(2)
+    s.setOption("graph", 0 if a.no_graph else 1)
+    s.setOption("fused_scans", 1 if a.fused_scans else 0)
     s.marchTimeStep(a.warmup)
     t0 = time.perf_counter()
     s.marchTimeStep(a.steps)  # returns after a device synchronisation
@@ -131,7 +135,8 @@ def main():
         "data": "synthetic (seeded freestream + perturbation initial field)",
         "config": {"workload": a.workload, "grid": [cfg.NI, cfg.NJ, cfg.NK], "precision": prec,
                    "ghosts": g, "filter_nodes": f, "stage_kernel": a.kernel,
-                   "tile_rows_x_zplanes": list(tile) if tile else None},
+                   "tile_rows_x_zplanes": list(tile) if tile else None,
+                   "hip_graph": not a.no_graph},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1) if achieved else None,
