@@ -434,21 +434,39 @@ void Case::stepOnce() {
     T* in[4] = {u, s1, s2, s1};
     T* out[4] = {s1, s2, s1, a};
     const double frac[4] = {0.5, 0.5, 1.0, 1.0};
+    // periodic density filter (src/Solver.cpp:134): decided on the device from t and timeLevel
+    const bool rhoFilter = !filterDisabled();
+    T* fRho = s2;  // S2's density field is free until stage 2
+    const int32_t* fFlag = &scal->filterNow;
+    if (rhoFilter) {
+        FilterSchedule fs{};
+        fs.n_intervals = cfg.n_filter_intervals;
+        fs.n_change = cfg.n_filter_change_times;
+        for (int q = 0; q < NS_MAX_FILTER; ++q) {
+            fs.intervals[q] = cfg.filter_intervals[q];
+            fs.change_times[q] = cfg.filter_change_times[q];
+        }
+        launchFilterFlag(stream, fs, scal);
+        launchRhoFilter<T>(stream, L, u, fRho, cls, fFlag);
+    }
     for (int st = 1; st <= 4; ++st) {
         if (stageKernel == 1) {
             mark(0, true);
             nStage4Partials = launchStageFused<T>(stream, st, L, P, W, in[st - 1], u, out[st - 1], a, cls,
                                                   (const T*)rhoPre, (const T*)rhoEPre, dtdev, tile,
-                                                  fusedScans ? partials : nullptr, cfg.conv_history > 0);
+                                                  fusedScans ? partials : nullptr, cfg.conv_history > 0,
+                                                  rhoFilter ? fRho : nullptr, rhoFilter ? fFlag : nullptr);
             mark(0, false);
         } else {
             mark(1, true);
             launchDerive<T>(stream, L, P, in[st - 1], pr, cls, W, (const T*)rhoPre, (const T*)rhoEPre);
             mark(1, false);
             mark(0, true);
-            launchRhsPlain<T>(stream, st, L, P, in[st - 1], pr, u, out[st - 1], a, cls, dtdev);
+            launchRhsPlain<T>(stream, st, L, P, in[st - 1], pr, u, out[st - 1], a, cls, dtdev, rhoFilter ? fRho : nullptr,
+                              rhoFilter ? fFlag : nullptr);
             mark(0, false);
         }
+        if (st == 1 && rhoFilter) launchRhoCommit<T>(stream, L, u, fRho, a, cls, fFlag);
         mark(2, true);
         applyBCs<T>(out[st - 1], frac[st - 1]);
         mark(2, false);
@@ -525,11 +543,11 @@ double Case::autotune(int reps) {
                     if (fp32)
                         launchStageFused<float>(stream, 2, L, P, W, (const float*)U, (const float*)U, (float*)S2, (float*)A,
                                                 cls, (const float*)rhoPre, (const float*)rhoEPre, &scal->dt, tl,
-                                                nullptr, 0);
+                                                nullptr, 0, nullptr, nullptr);
                     else
                         launchStageFused<double>(stream, 2, L, P, W, (const double*)U, (const double*)U, (double*)S2,
                                                  (double*)A, cls, (const double*)rhoPre, (const double*)rhoEPre,
-                                                 &scal->dt, tl, nullptr, 0);
+                                                 &scal->dt, tl, nullptr, 0, nullptr, nullptr);
                 }
             };
             run(1);  // warm-up
@@ -653,7 +671,6 @@ void Case::dropGraphs() {
 
 void Case::step(int nsteps) {
     if (!initialized) throw StateError("step before init");
-    if (!filterDisabled()) throw Unsupported("the periodic density filter (filterIntervals) is not implemented on the device");
     int done = 0;
     while (done < nsteps) {
         int batch = std::min(nsteps - done, histCap);

@@ -19,7 +19,14 @@ struct StepScalars {
     uint64_t level;
     uint64_t histCount;
     int32_t nonFinite;
-    int32_t pad;
+    int32_t filterNow;  // periodic density filter active in the current step
+};
+
+// Schedule of the periodic density filter (Mesh::checkFilterCondition, src/Mesh.cpp:85-103).
+struct FilterSchedule {
+    int n_intervals, n_change;
+    uint32_t intervals[16];
+    double change_times[16];
 };
 
 // ---- unfused stage (stage_plain.hip)
@@ -28,7 +35,18 @@ void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T
                   const T* rhoPre, const T* rhoEPre);
 template <class T>
 void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
-                    T* Sout, T* A, const uint8_t* cls, const double* dtdev);
+                    T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* filteredRho,
+                    const int32_t* filterFlag);
+
+// Periodic density filter (Mesh::applyFilter_ifAppropriate, src/Mesh.cpp:108-140), in three parts:
+// the flag for this step from (t, timeLevel); the filtered interior density of U into `out`; after the
+// stage-1 kernel, U's interior density := out and U's untouched solid density := A's.
+void launchFilterFlag(hipStream_t s, const FilterSchedule& fs, StepScalars* st);
+template <class T>
+void launchRhoFilter(hipStream_t s, const Layout& L, const T* U, T* out, const uint8_t* cls, const int32_t* flag);
+template <class T>
+void launchRhoCommit(hipStream_t s, const Layout& L, T* U, const T* filtered, const T* A, const uint8_t* cls,
+                     const int32_t* flag);
 
 // ---- fused stage (stage_fused.hip): primitives + RHS + combine in one pass
 // Tile shape of the fused stage: ty output rows per block (one of fusedTileHeights), kc z-planes per
@@ -40,10 +58,13 @@ struct FusedTile {
 // Stage 4 with partials != null also writes one partial per block of the end-of-step scans over
 // interior, non-filter nodes (norms of change when norms != 0, dt maxima, non-finite count). Returns the
 // number of blocks (= partial slots written).
+// filteredRho / filterFlag (stage 1): when *filterFlag, interior nodes combine from filteredRho instead of
+// the stage input's density, and non-interior nodes copy their density from the stage input.
 template <class T>
 int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
-                     const double* dtdev, FusedTile tile, double* partials, int norms);
+                     const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
+                     const int32_t* filterFlag);
 const int* fusedTileHeights(int* n);
 int fusedMaxBlocks(const Layout& L);  // largest block count over all tile shapes
 

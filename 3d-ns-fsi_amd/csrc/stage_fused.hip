@@ -68,6 +68,9 @@ struct FusedArgs {
     double* part;
     int norms;
     T vm;  // max(4/3, gamma/Pr)
+    // stage 1 of a step with the periodic density filter: combine base density and flag
+    const T* fRho;
+    const int32_t* fFlag;
 };
 
 // Whole-wavefront shifts by one lane (DPP wave_shl:1 / wave_shr:1 of GFX9): shl1 gives each lane the
@@ -274,6 +277,7 @@ void k_stage_fused(FusedArgs<T> a) {
     const bool outLane = lane >= 1 && lane <= 62 && i < L.NI;
     const Coef<T>& c = a.c;
     const double dtd = *a.dtdev;
+    const bool rhoFiltered = STAGE == 1 && a.fFlag && *a.fFlag;
 
     // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
     Pt<T> qm, q0, q1;
@@ -381,10 +385,14 @@ void k_stage_fused(FusedArgs<T> a) {
                 rhs[q] += phiz[q] - phizm[q];
             }
             if (doOut) {
-                const T cur[5] = {q0.rho, q0.ru, q0.rv, q0.rw, q0.E};
+                T cur[5] = {q0.rho, q0.ru, q0.rv, q0.rw, q0.E};
                 if (interior) {
                     if (STAGE == 1) {
                         const T h2 = (T)(dtd / 2), h6 = (T)(dtd / 6);
+                        // the reference filters the density of U before the step but keeps the
+                        // primitives, so only the combine base changes (src/Solver.cpp:134-141)
+                        const T c0 = cur[0];
+                        cur[0] = rhoFiltered ? ldAt(a.fRho, bo) : c0;
                         for (int q = 0; q < 5; ++q) {
                             stAt(a.Sout[q], bo, cur[q] + h2 * rhs[q]);
                             stAt(a.A[q], bo, cur[q] + h6 * rhs[q]);
@@ -422,8 +430,12 @@ void k_stage_fused(FusedArgs<T> a) {
                         }
                     }
                 } else {
-                    if (STAGE == 1)
+                    if (STAGE == 1) {
+                        // the reference's target array holds the unfiltered density of U after the
+                        // filter's buffer swap (src/Mesh.cpp:133), the other fields of the step before
+                        if (rhoFiltered) acc[0] = cur[0];
                         for (int q = 0; q < 5; ++q) stAt(a.Sout[q], bo, acc[q]);
+                    }
                     else if (STAGE == 4)
                         for (int q = 0; q < 5; ++q) stAt(a.A[q], bo, cur[q]);
                     else
@@ -490,8 +502,11 @@ void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a0, int kc) {
 template <class T>
 int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
-                     const double* dtdev, FusedTile tile, double* partials, int norms) {
+                     const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
+                     const int32_t* filterFlag) {
     FusedArgs<T> a{};
+    a.fRho = stage == 1 ? filteredRho : nullptr;
+    a.fFlag = stage == 1 ? filterFlag : nullptr;
     a.part = stage == 4 ? partials : nullptr;
     a.norms = norms;
     a.vm = (T)std::max(4. / 3, P.gamma / P.Pr);
@@ -543,9 +558,9 @@ const int* fusedTileHeights(int* n) {
 
 template int launchStageFused<double>(hipStream_t, int, const Layout&, const Params&, const Window&, const double*,
                                       const double*, double*, double*, const uint8_t*, const double*, const double*,
-                                      const double*, FusedTile, double*, int);
+                                      const double*, FusedTile, double*, int, const double*, const int32_t*);
 template int launchStageFused<float>(hipStream_t, int, const Layout&, const Params&, const Window&, const float*,
                                      const float*, float*, float*, const uint8_t*, const float*, const float*,
-                                     const double*, FusedTile, double*, int);
+                                     const double*, FusedTile, double*, int, const float*, const int32_t*);
 
 }  // namespace nsfsi

@@ -53,7 +53,8 @@ __global__ void k_derive(Layout L, Params P, const T* S, T* prim, const uint8_t*
 
 template <class T, int STAGE>
 __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, const T* U, T* Sout, T* A, const uint8_t* cls,
-                            const double* dtdev) {
+                            const double* dtdev, const T* fRho, const int32_t* fFlag) {
+    const bool rhoFiltered = STAGE == 1 && fFlag && *fFlag;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y, k = blockIdx.z;
     if (i >= L.NI) return;
@@ -63,6 +64,7 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
     if ((cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) {
         const T* src = STAGE == 1 ? A : Sin;
         for (int q = 0; q < 5; ++q) out[o + q * F] = src[o + q * F];
+        if (rhoFiltered) out[o] = Sin[o];  // unfiltered density (see the fused kernel)
         return;
     }
     const T dx = (T)P.hx, dy = (T)P.hy, dz = (T)P.hz;
@@ -169,7 +171,7 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
     if (STAGE == 1) {
         const T h2 = (T)(dtd / 2), h6 = (T)(dtd / 6);
         for (int q = 0; q < 5; ++q) {
-            T u0 = U[o + q * F];
+            T u0 = (q == 0 && rhoFiltered) ? fRho[o] : U[o + q * F];
             Sout[o + q * F] = u0 + h2 * kq[q];
             A[o + q * F] = u0 + h6 * kq[q];
         }
@@ -196,23 +198,97 @@ void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T
 
 template <class T>
 void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
-                    T* Sout, T* A, const uint8_t* cls, const double* dtdev) {
+                    T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag) {
     dim3 blk(64), grd((L.NI + 63) / 64, L.NJ, L.NK);
     switch (stage) {
-        case 1: hipLaunchKernelGGL((k_rhs_plain<T, 1>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev); break;
-        case 2: hipLaunchKernelGGL((k_rhs_plain<T, 2>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev); break;
-        case 3: hipLaunchKernelGGL((k_rhs_plain<T, 3>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev); break;
-        default: hipLaunchKernelGGL((k_rhs_plain<T, 4>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev); break;
+        case 1:
+            hipLaunchKernelGGL((k_rhs_plain<T, 1>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_rhs_plain<T, 2>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr, nullptr);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_rhs_plain<T, 3>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr, nullptr);
+            break;
+        default:
+            hipLaunchKernelGGL((k_rhs_plain<T, 4>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr, nullptr);
+            break;
     }
 }
+
+namespace {
+
+__global__ void k_filter_flag(FilterSchedule fs, StepScalars* st) {
+    // Mesh::checkFilterCondition, src/Mesh.cpp:85-103, with t and timeLevel at the start of the step
+    int counter = 0;
+    for (int q = 0; q < fs.n_change; ++q)
+        if (st->t > fs.change_times[q]) ++counter;
+    int now = 0;
+    if (counter < fs.n_intervals) {
+        uint32_t iv = fs.intervals[counter];
+        if (iv > 0 && (st->level + 1) % iv == 0) now = 1;
+    }
+    st->filterNow = now;
+}
+
+template <class T>
+__global__ void k_rho_filter(Layout L, const T* U, T* out, const uint8_t* cls, const int32_t* flag) {
+    if (!*flag) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.NI) return;
+    size_t o = L.off(i, j, k);
+    if ((cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) return;
+    const long X = 1, Y = L.pitch, Z = (long)L.plane;
+    // (6 rho_P + sum of the 6 face neighbours) / 12, in the reference's summation order
+    out[o] = ((T)6 * U[o] + U[o + X] + U[o - X] + U[o + Y] + U[o - Y] + U[o + Z] + U[o - Z]) / (T)12;
+}
+
+template <class T>
+__global__ void k_rho_commit(Layout L, T* U, const T* filtered, const T* A, const uint8_t* cls, const int32_t* flag) {
+    if (!*flag) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.NI) return;
+    size_t o = L.off(i, j, k);
+    uint8_t c = cls[o];
+    if ((c & CLS_TYPE_MASK) == NS_FLUID_INTERIOR) U[o] = filtered[o];
+    else if (c & CLS_INACTIVE_IC) U[o] = A[o];  // the swapped-in scratch array keeps its solid values
+}
+
+}  // namespace
+
+void launchFilterFlag(hipStream_t s, const FilterSchedule& fs, StepScalars* st) {
+    hipLaunchKernelGGL(k_filter_flag, dim3(1), dim3(1), 0, s, fs, st);
+}
+
+template <class T>
+void launchRhoFilter(hipStream_t s, const Layout& L, const T* U, T* out, const uint8_t* cls, const int32_t* flag) {
+    dim3 blk(64), grd((L.NI + 63) / 64, L.NJ, L.NK);
+    hipLaunchKernelGGL(k_rho_filter<T>, grd, blk, 0, s, L, U, out, cls, flag);
+}
+
+template <class T>
+void launchRhoCommit(hipStream_t s, const Layout& L, T* U, const T* filtered, const T* A, const uint8_t* cls,
+                     const int32_t* flag) {
+    dim3 blk(64), grd((L.NI + 63) / 64, L.NJ, L.NK);
+    hipLaunchKernelGGL(k_rho_commit<T>, grd, blk, 0, s, L, U, filtered, A, cls, flag);
+}
+
+template void launchRhoFilter<double>(hipStream_t, const Layout&, const double*, double*, const uint8_t*, const int32_t*);
+template void launchRhoFilter<float>(hipStream_t, const Layout&, const float*, float*, const uint8_t*, const int32_t*);
+template void launchRhoCommit<double>(hipStream_t, const Layout&, double*, const double*, const double*, const uint8_t*,
+                                      const int32_t*);
+template void launchRhoCommit<float>(hipStream_t, const Layout&, float*, const float*, const float*, const uint8_t*,
+                                     const int32_t*);
 
 template void launchDerive<double>(hipStream_t, const Layout&, const Params&, const double*, double*, const uint8_t*,
                                    const Window&, const double*, const double*);
 template void launchDerive<float>(hipStream_t, const Layout&, const Params&, const float*, float*, const uint8_t*,
                                   const Window&, const float*, const float*);
 template void launchRhsPlain<double>(hipStream_t, int, const Layout&, const Params&, const double*, const double*,
-                                     const double*, double*, double*, const uint8_t*, const double*);
+                                     const double*, double*, double*, const uint8_t*, const double*, const double*,
+                                     const int32_t*);
 template void launchRhsPlain<float>(hipStream_t, int, const Layout&, const Params&, const float*, const float*,
-                                    const float*, float*, float*, const uint8_t*, const double*);
+                                    const float*, float*, float*, const uint8_t*, const double*, const float*,
+                                    const int32_t*);
 
 }  // namespace nsfsi

@@ -4,7 +4,7 @@ Integer outputs (node classes, ghost and filter sets, image-point boxes) must ma
 FP64 geometry (body intercept and image points) and the initial state must match bit for bit; the FP64
 trajectory must match to <= 1e-12 relative L2 per conserved field after N steps, or, where the
 reference algorithm itself is more sensitive than that, to within 4x the change the oracle's own
-trajectory shows under a one-ulp perturbation of its input (measured in the same test).
+trajectory shows under a one-ulp perturbation of its state (measured in the same test).
 """
 import numpy as np
 import pytest
@@ -19,21 +19,24 @@ TOL_FP64 = 1e-12
 
 
 def ulp_sensitivity(cfg, steps):
-    """Relative L2 change of the oracle's own trajectory when rho*u of the initial state moves by one ulp
-    (random sign per node). The reference interpolates ghost values with 8x8 Vandermonde systems in global
+    """Relative L2 change of the oracle's own trajectory when, after the first step, every stored node value
+    (conserved, primitive and transport fields) moves by one ulp (random sign per node). The perturbation
+    is applied after a step because some cases start from an all-zero perturbation state, where one ulp is
+    a denormal. The reference interpolates ghost values with 8x8 Vandermonde systems in global
     coordinates (src/Boundary.cpp:788-833), whose condition number (~1e5 at 48^3) amplifies rounding-level
     input differences; no implementation that is not bit-identical can agree more closely than this."""
     a = oracle.Oracle(cfg)
     a.initialize()
     b = oracle.Oracle(cfg)
     b.initialize()
+    a.march(1)
+    b.march(1)
     rng = np.random.default_rng(0)
-    ru = b.conserved()[1]
-    ru2 = np.where(rng.random(ru.shape) < 0.5, np.nextafter(ru, np.inf), np.nextafter(ru, -np.inf))
-    b.set_field("rho_u", ru2)
-    b.derive_all()
-    a.march(steps)
-    b.march(steps)
+    for w in range(12):
+        f = b.field(w)
+        b.set_field(w, np.where(rng.random(f.shape) < 0.5, np.nextafter(f, np.inf), np.nextafter(f, -np.inf)))
+    a.march(steps - 1)
+    b.march(steps - 1)
     A, B = a.conserved(), b.conserved()
     return [rel_l2(B[q], A[q]) for q in range(5)]
 
@@ -147,3 +150,64 @@ def test_fused_matches_unfused_fp32():
         s.close()
     for q in range(5):
         assert rel_l2(out[0][q], out[1][q]) < 1e-5
+
+
+def _traj_check(cfg, steps, s=None, o=None):
+    if s is None:
+        s, o = _pair(cfg)
+        s.marchTimeStep(steps)
+        o.march(steps)
+    U, V = s.mesh.conserved(), o.conserved()
+    errs = [rel_l2(U[q], V[q]) for q in range(5)]
+    floor = ulp_sensitivity(cfg, steps)
+    print("rel L2 per field", errs, "oracle 1-ulp sensitivity", floor)
+    for q in range(5):
+        assert errs[q] <= max(TOL_FP64, 4 * floor[q]), (q, errs[q], floor[q])
+    return s, o
+
+
+@pytest.mark.parametrize("kernel", [1, 0], ids=["fused", "unfused"])
+def test_reference_default_case_with_density_filter(kernel):
+    """The reference's shipped ConfigFile case (161x83x3 quasi-2-D cylinder channel, periodic z, filter
+    every 50 steps) for 60 steps: the periodic density filter fires at step 50."""
+    cfg = cases.reference_default()
+    cfg.stop_criterion = 0
+    cfg.stop_time_level = 60
+    s, o = _pair(cfg)
+    s.setOption("stage_kernel", kernel)
+    s.marchTimeStep(60)
+    o.march(60)
+    assert s.time()[1] == o.state()[1] == 60
+    assert abs(s.time()[0] - o.state()[0]) <= 1e-12 * o.state()[0]
+    _traj_check(cfg, 60, s, o)
+
+
+def test_density_filter_every_three_steps():
+    cfg = cases.sphere48(N=24, steps=10)
+    cfg.n_filter_intervals = 1
+    cfg.filter_intervals[0] = 3
+    cfg.n_filter_change_times = 0
+    _, o = _traj_check(cfg, 10)
+    # the filter must matter at this tolerance: without it the oracle's state moves by > 1e-3
+    cfg.filter_intervals[0] = 0
+    u = oracle.Oracle(cfg)
+    u.initialize()
+    u.march(10)
+    assert min(rel_l2(u.conserved()[q], o.conserved()[q]) for q in range(5)) > 1e-3
+
+
+def test_run_to_end_time():
+    """Simulation::run with stopCriterion end_time: the last step is shortened to hit t_end exactly."""
+    cfg = cases.sphere48(N=24, steps=1)
+    cfg.stop_criterion = 1
+    cfg.t_end = 0.3
+    sim = nsfsi.Simulation(cfg)
+    t, lv = sim.run()
+    o = oracle.Oracle(cfg)
+    o.initialize()
+    o.run()
+    ot, olv, _ = o.state()
+    assert lv == olv and t == pytest.approx(ot, rel=1e-14) and abs(t - 0.3) < 1e-14
+    U, V = sim.solver.mesh.conserved(), o.conserved()
+    for q in range(5):
+        assert rel_l2(U[q], V[q]) < 1e-11
