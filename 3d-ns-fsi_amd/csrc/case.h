@@ -49,7 +49,7 @@ public:
     // Run-time options: "stage_kernel" 0 = unfused (primitive pass + per-node RHS), 1 = fused.
     void setOption(const std::string& name, int64_t value);
     int stageKernel = 1;
-    bool fusedScans = false;  // end-of-step scans inside the stage-4 kernel (else a separate full pass)
+    bool fusedScans = true;  // end-of-step scans inside the stage-4 kernel (else a separate full pass)
     FusedTile tile{};
     // Times every fused-stage tile shape on this case (stage-2 launches into scratch outputs, state
     // untouched) and keeps the fastest; returns the chosen tile and its time per launch in ms.
@@ -84,6 +84,7 @@ public:
     // fields (element type float or double)
     void *U = nullptr, *A = nullptr, *S1 = nullptr, *S2 = nullptr, *prim = nullptr;
     void *rhoPre = nullptr, *rhoEPre = nullptr, *ftmp = nullptr;
+    uint16_t* comp = nullptr;  // FP32 compensated combine: carried rounding error, 5 bfloat16 fields
 
     // step scalars and reductions
     StepScalars* scal = nullptr;

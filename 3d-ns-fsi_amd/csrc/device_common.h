@@ -57,6 +57,26 @@ __device__ __forceinline__ void deriveCons(const Eos<T>& e, T u, T v, T w, T p, 
     rE = p / e.gm1 + m / (T)2 * (u * u + v * v + w * w);
 }
 
+// Compensated FP32 combine. The increment form keeps Uⁿ out of the stage accumulator (it holds
+// D = dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4 only), so the one rounding that is relative to |U| happens
+// in U + D at the end of the step. TwoSum (Knuth) gives that rounding error exactly; it is carried to
+// the next step's increment (Kahan's compensated summation over the time steps) in a bfloat16 buffer:
+// the error is at most half an ulp of U, and 8 significant bits of it leave a residual of 2^-9 of that.
+__device__ __forceinline__ float bf16Decode(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t bf16Encode(float x) {
+    // round to nearest, ties to even (the compensation is finite wherever the state is)
+    uint32_t b = __float_as_uint(x);
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+// s + e == u + d exactly (s = fl(u + d)); no operand order assumption.
+__device__ __forceinline__ float twoSum(float u, float d, float& e) {
+    float s = u + d;
+    float bv = s - u;
+    e = (u - (s - bv)) + (d - bv);
+    return s;
+}
+
 // Order-preserving encoding of a non-negative double for atomicMax on the bit pattern.
 __device__ __forceinline__ unsigned long long orderedBits(double x) { return (unsigned long long)__double_as_longlong(x); }
 

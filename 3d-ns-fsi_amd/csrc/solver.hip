@@ -306,6 +306,7 @@ void Case::allocFields() {
     S1 = alloc(fb);
     S2 = alloc(fb);
     prim = alloc(7 * L.fstride * sizeof(T));
+    if (fp32 && cfg.kahan) comp = (uint16_t*)alloc(5 * L.fstride * sizeof(uint16_t));
     hipCheck(hipMemsetAsync(U, 0, fb, stream), "memset");
     hipCheck(hipMemsetAsync(A, 0, fb, stream), "memset");  // the reference's zero-initialised "Old" arrays
     hipCheck(hipMemsetAsync(S1, 0, fb, stream), "memset");
@@ -384,6 +385,7 @@ void Case::hostInitialField(std::vector<T>& h) const {
 template <class T>
 void Case::uploadState(const std::vector<T>& h) {
     hipCheck(hipMemcpyAsync(U, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream), "upload IC");
+    if (comp) hipCheck(hipMemsetAsync(comp, 0, 5 * L.fstride * sizeof(uint16_t), stream), "memset");
     // pre-filter side tables start as the current values (the reference's primitives match the IC)
     launchFilterCompute<T>(stream, L, W, filterList, nFilter, (const T*)U, (T*)ftmp, (T*)rhoPre, (T*)rhoEPre);
     if (nInactive > 0)
@@ -455,7 +457,7 @@ void Case::stepOnce() {
             nStage4Partials = launchStageFused<T>(stream, st, L, P, W, in[st - 1], u, out[st - 1], a, cls,
                                                   (const T*)rhoPre, (const T*)rhoEPre, dtdev, tile,
                                                   fusedScans ? partials : nullptr, cfg.conv_history > 0,
-                                                  rhoFilter ? fRho : nullptr, rhoFilter ? fFlag : nullptr);
+                                                  rhoFilter ? fRho : nullptr, rhoFilter ? fFlag : nullptr, comp);
             mark(0, false);
         } else {
             mark(1, true);
@@ -463,7 +465,7 @@ void Case::stepOnce() {
             mark(1, false);
             mark(0, true);
             launchRhsPlain<T>(stream, st, L, P, in[st - 1], pr, u, out[st - 1], a, cls, dtdev, rhoFilter ? fRho : nullptr,
-                              rhoFilter ? fFlag : nullptr);
+                              rhoFilter ? fFlag : nullptr, comp);
             mark(0, false);
         }
         if (st == 1 && rhoFilter) launchRhoCommit<T>(stream, L, u, fRho, a, cls, fFlag);
@@ -543,11 +545,11 @@ double Case::autotune(int reps) {
                     if (fp32)
                         launchStageFused<float>(stream, 2, L, P, W, (const float*)U, (const float*)U, (float*)S2, (float*)A,
                                                 cls, (const float*)rhoPre, (const float*)rhoEPre, &scal->dt, tl,
-                                                nullptr, 0, nullptr, nullptr);
+                                                nullptr, 0, nullptr, nullptr, comp);
                     else
                         launchStageFused<double>(stream, 2, L, P, W, (const double*)U, (const double*)U, (double*)S2,
                                                  (double*)A, cls, (const double*)rhoPre, (const double*)rhoEPre,
-                                                 &scal->dt, tl, nullptr, 0, nullptr, nullptr);
+                                                 &scal->dt, tl, nullptr, 0, nullptr, nullptr, nullptr);
                 }
             };
             run(1);  // warm-up

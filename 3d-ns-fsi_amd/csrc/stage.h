@@ -36,7 +36,7 @@ void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T
 template <class T>
 void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                     T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* filteredRho,
-                    const int32_t* filterFlag);
+                    const int32_t* filterFlag, uint16_t* comp);
 
 // Periodic density filter (Mesh::applyFilter_ifAppropriate, src/Mesh.cpp:108-140), in three parts:
 // the flag for this step from (t, timeLevel); the filtered interior density of U into `out`; after the
@@ -60,11 +60,14 @@ struct FusedTile {
 // number of blocks (= partial slots written).
 // filteredRho / filterFlag (stage 1): when *filterFlag, interior nodes combine from filteredRho instead of
 // the stage input's density, and non-interior nodes copy their density from the stage input.
+// comp != null (FP32 only; both stage kernels): compensated combine. The accumulator A holds the step's
+// increment at interior nodes (stages 1-3) and stage 4 forms U + increment with the carried rounding
+// error; comp is 5 bfloat16 fields with the layout's element offsets, zero before the first step.
 template <class T>
 int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
                      const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
-                     const int32_t* filterFlag);
+                     const int32_t* filterFlag, uint16_t* comp);
 const int* fusedTileHeights(int* n);
 int fusedMaxBlocks(const Layout& L);  // largest block count over all tile shapes
 

@@ -71,6 +71,9 @@ struct FusedArgs {
     // stage 1 of a step with the periodic density filter: combine base density and flag
     const T* fRho;
     const int32_t* fFlag;
+    // compensated FP32 combine: per-field bfloat16 rounding error of U carried between steps (the
+    // stage accumulator then holds the increment only)
+    uint16_t* comp[5];
 };
 
 // Whole-wavefront shifts by one lane (DPP wave_shl:1 / wave_shr:1 of GFX9): shl1 gives each lane the
@@ -256,7 +259,7 @@ __device__ __forceinline__ void readFaceNb(const Smem<T, TY>& sm, int r, int lan
 }
 
 // FP32: at least 4 waves per SIMD (<= 128 VGPRs); FP64: at least 2.
-template <class T, int STAGE, int TY, bool SCAN>
+template <class T, int STAGE, int TY, bool SCAN, bool KAHAN>
 __global__ __launch_bounds__(64 * (TY + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void k_stage_fused(FusedArgs<T> a) {
     __shared__ Smem<T, TY> sm;
@@ -278,6 +281,7 @@ void k_stage_fused(FusedArgs<T> a) {
     const Coef<T>& c = a.c;
     const double dtd = *a.dtdev;
     const bool rhoFiltered = STAGE == 1 && a.fFlag && *a.fFlag;
+    static_assert(!KAHAN || sizeof(T) == 4, "the compensated combine is the FP32 form");
 
     // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
     Pt<T> qm, q0, q1;
@@ -322,7 +326,7 @@ void k_stage_fused(FusedArgs<T> a) {
                 interior = (cl & CLS_TYPE_MASK) == NS_FLUID_INTERIOR;
                 filterNode = (cl & CLS_FILTER) != 0;
             }
-            if (STAGE == 2 || STAGE == 3 || (SCAN && a.norms)) {
+            if (STAGE == 2 || STAGE == 3 || (STAGE == 4 && KAHAN) || (SCAN && a.norms)) {
                 if (interior)
                     for (int q = 0; q < 5; ++q) base[q] = ldAt(a.U[q], bo);
             }
@@ -395,7 +399,7 @@ void k_stage_fused(FusedArgs<T> a) {
                         cur[0] = rhoFiltered ? ldAt(a.fRho, bo) : c0;
                         for (int q = 0; q < 5; ++q) {
                             stAt(a.Sout[q], bo, cur[q] + h2 * rhs[q]);
-                            stAt(a.A[q], bo, cur[q] + h6 * rhs[q]);
+                            stAt(a.A[q], bo, KAHAN ? h6 * rhs[q] : cur[q] + h6 * rhs[q]);
                         }
                     } else if (STAGE == 2 || STAGE == 3) {
                         const T hs = (T)(STAGE == 2 ? dtd / 2 : dtd), h3 = (T)(dtd / 3);
@@ -408,6 +412,14 @@ void k_stage_fused(FusedArgs<T> a) {
                         T an[5];
                         for (int q = 0; q < 5; ++q) {
                             an[q] = acc[q] + h6 * rhs[q];
+                            if constexpr (KAHAN) {
+                                // U^{n+1} = U^n + (D + e^{n-1}), its rounding error kept for the next step
+                                const uint32_t ho = bo >> 1;
+                                uint16_t* cq = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(a.comp[q]) + ho);
+                                T e;
+                                an[q] = twoSum(base[q], an[q] + bf16Decode(*cq), e);
+                                *cq = bf16Encode(e);
+                            }
                             stAt(a.A[q], bo, an[q]);
                         }
                         // end-of-step scans; filter nodes change once more in the boundary pass and are
@@ -476,8 +488,8 @@ int fusedBlocks(const Layout& L, int kc) {
     return ((L.NI + 61) / 62) * ((L.NJ + TY - 1) / TY) * ((L.NK + kc - 1) / kc);
 }
 
-template <class T, int TY>
-void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a0, int kc) {
+template <class T, int TY, bool K>
+void launchTYK(hipStream_t s, int stage, const FusedArgs<T>& a0, int kc) {
     FusedArgs<T> a = a0;
     a.nbx = (a.L.NI + 61) / 62;
     a.nby = (a.L.NJ + TY - 1) / TY;
@@ -487,14 +499,25 @@ void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a0, int kc) {
     const int grid = (a.nblocks + 7) / 8 * 8;
     dim3 blk(64, TY + 2);
     switch (stage) {
-        case 1: hipLaunchKernelGGL((k_stage_fused<T, 1, TY, false>), dim3(grid), blk, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_stage_fused<T, 2, TY, false>), dim3(grid), blk, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage_fused<T, 3, TY, false>), dim3(grid), blk, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_stage_fused<T, 1, TY, false, K>), dim3(grid), blk, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_stage_fused<T, 2, TY, false, K>), dim3(grid), blk, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage_fused<T, 3, TY, false, K>), dim3(grid), blk, 0, s, a); break;
         default:
-            if (a.part) hipLaunchKernelGGL((k_stage_fused<T, 4, TY, true>), dim3(grid), blk, 0, s, a);
-            else hipLaunchKernelGGL((k_stage_fused<T, 4, TY, false>), dim3(grid), blk, 0, s, a);
+            if (a.part) hipLaunchKernelGGL((k_stage_fused<T, 4, TY, true, K>), dim3(grid), blk, 0, s, a);
+            else hipLaunchKernelGGL((k_stage_fused<T, 4, TY, false, K>), dim3(grid), blk, 0, s, a);
             break;
     }
+}
+
+template <class T, int TY>
+void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a, int kc) {
+    if constexpr (sizeof(T) == 4) {
+        if (a.comp[0]) {
+            launchTYK<T, TY, true>(s, stage, a, kc);
+            return;
+        }
+    }
+    launchTYK<T, TY, false>(s, stage, a, kc);
 }
 
 }  // namespace
@@ -503,8 +526,9 @@ template <class T>
 int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
                      const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
-                     const int32_t* filterFlag) {
+                     const int32_t* filterFlag, uint16_t* comp) {
     FusedArgs<T> a{};
+    if (comp && sizeof(T) != 4) throw std::invalid_argument("compensated combine needs FP32 fields");
     a.fRho = stage == 1 ? filteredRho : nullptr;
     a.fFlag = stage == 1 ? filterFlag : nullptr;
     a.part = stage == 4 ? partials : nullptr;
@@ -534,6 +558,7 @@ int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P,
         a.U[q] = U + q * L.fstride;
         a.Sout[q] = Sout ? Sout + q * L.fstride : nullptr;
         a.A[q] = A + q * L.fstride;
+        a.comp[q] = comp ? comp + q * L.fstride : nullptr;
     }
     a.cls = cls;
     a.rhoPre = rhoPre;
@@ -558,9 +583,11 @@ const int* fusedTileHeights(int* n) {
 
 template int launchStageFused<double>(hipStream_t, int, const Layout&, const Params&, const Window&, const double*,
                                       const double*, double*, double*, const uint8_t*, const double*, const double*,
-                                      const double*, FusedTile, double*, int, const double*, const int32_t*);
+                                      const double*, FusedTile, double*, int, const double*, const int32_t*,
+                                      uint16_t*);
 template int launchStageFused<float>(hipStream_t, int, const Layout&, const Params&, const Window&, const float*,
                                      const float*, float*, float*, const uint8_t*, const float*, const float*,
-                                     const double*, FusedTile, double*, int, const float*, const int32_t*);
+                                     const double*, FusedTile, double*, int, const float*, const int32_t*,
+                                     uint16_t*);
 
 }  // namespace nsfsi

@@ -19,6 +19,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "device_common.h"
 #include "stage.h"
 
@@ -51,9 +53,9 @@ __global__ void k_derive(Layout L, Params P, const T* S, T* prim, const uint8_t*
     prim[o + 6 * L.fstride] = mu * e.pf;
 }
 
-template <class T, int STAGE>
+template <class T, int STAGE, bool KAHAN>
 __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, const T* U, T* Sout, T* A, const uint8_t* cls,
-                            const double* dtdev, const T* fRho, const int32_t* fFlag) {
+                            const double* dtdev, const T* fRho, const int32_t* fFlag, uint16_t* comp) {
     const bool rhoFiltered = STAGE == 1 && fFlag && *fFlag;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y, k = blockIdx.z;
@@ -173,7 +175,7 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         for (int q = 0; q < 5; ++q) {
             T u0 = (q == 0 && rhoFiltered) ? fRho[o] : U[o + q * F];
             Sout[o + q * F] = u0 + h2 * kq[q];
-            A[o + q * F] = u0 + h6 * kq[q];
+            A[o + q * F] = KAHAN ? h6 * kq[q] : u0 + h6 * kq[q];
         }
     } else if (STAGE == 2 || STAGE == 3) {
         const T hs = (T)(STAGE == 2 ? dtd / 2 : dtd), h3 = (T)(dtd / 3);
@@ -183,7 +185,16 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         }
     } else {
         const T h6 = (T)(dtd / 6);
-        for (int q = 0; q < 5; ++q) A[o + q * F] = A[o + q * F] + h6 * kq[q];
+        for (int q = 0; q < 5; ++q) {
+            T an = A[o + q * F] + h6 * kq[q];
+            if constexpr (KAHAN) {
+                // U^{n+1} = U^n + (D + e^{n-1}); the new rounding error is carried (device_common.h)
+                T e;
+                an = twoSum(U[o + q * F], an + bf16Decode(comp[o + q * F]), e);
+                comp[o + q * F] = bf16Encode(e);
+            }
+            A[o + q * F] = an;
+        }
     }
 }
 
@@ -196,24 +207,44 @@ void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T
     hipLaunchKernelGGL(k_derive<T>, grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
 }
 
-template <class T>
-void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
-                    T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag) {
+template <class T, bool K>
+void launchRhsPlainK(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
+                     T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag,
+                     uint16_t* comp) {
     dim3 blk(64), grd((L.NI + 63) / 64, L.NJ, L.NK);
     switch (stage) {
         case 1:
-            hipLaunchKernelGGL((k_rhs_plain<T, 1>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag);
+            hipLaunchKernelGGL((k_rhs_plain<T, 1, K>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho,
+                               fFlag, comp);
             break;
         case 2:
-            hipLaunchKernelGGL((k_rhs_plain<T, 2>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr, nullptr);
+            hipLaunchKernelGGL((k_rhs_plain<T, 2, K>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr,
+                               nullptr, comp);
             break;
         case 3:
-            hipLaunchKernelGGL((k_rhs_plain<T, 3>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr, nullptr);
+            hipLaunchKernelGGL((k_rhs_plain<T, 3, K>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr,
+                               nullptr, comp);
             break;
         default:
-            hipLaunchKernelGGL((k_rhs_plain<T, 4>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr, nullptr);
+            hipLaunchKernelGGL((k_rhs_plain<T, 4, K>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr,
+                               nullptr, comp);
             break;
     }
+}
+
+template <class T>
+void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
+                    T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag,
+                    uint16_t* comp) {
+    if constexpr (sizeof(T) == 4) {
+        if (comp) {
+            launchRhsPlainK<T, true>(s, stage, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag, comp);
+            return;
+        }
+    } else {
+        if (comp) throw std::invalid_argument("compensated combine needs FP32 fields");
+    }
+    launchRhsPlainK<T, false>(s, stage, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag, nullptr);
 }
 
 namespace {
@@ -286,9 +317,9 @@ template void launchDerive<float>(hipStream_t, const Layout&, const Params&, con
                                   const Window&, const float*, const float*);
 template void launchRhsPlain<double>(hipStream_t, int, const Layout&, const Params&, const double*, const double*,
                                      const double*, double*, double*, const uint8_t*, const double*, const double*,
-                                     const int32_t*);
+                                     const int32_t*, uint16_t*);
 template void launchRhsPlain<float>(hipStream_t, int, const Layout&, const Params&, const float*, const float*,
                                     const float*, float*, float*, const uint8_t*, const double*, const float*,
-                                    const int32_t*);
+                                    const int32_t*, uint16_t*);
 
 }  // namespace nsfsi

@@ -71,8 +71,10 @@ def main():
     ap.add_argument("--N", type=int, default=None, help="override the grid edge")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="fused", choices=["fused", "unfused"])
-    ap.add_argument("--fused-scans", action="store_true", help="end-of-step scans inside the last stage kernel")
+    ap.add_argument("--fused-scans", type=int, default=None, choices=[0, 1],
+                    help="end-of-step scans inside the last stage kernel (default: the library's choice)")
     ap.add_argument("--no-graph", action="store_true", help="launch kernel by kernel instead of replaying a HIP graph")
+    ap.add_argument("--no-kahan", action="store_true", help="FP32: plain stage combine instead of the compensated one")
     ap.add_argument("--tile", default=None, help="fixed fused tile 'rows,zplanes' instead of autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB (counter calibration under rocprofv3)")
@@ -92,6 +94,7 @@ def main():
     prec = a.precision or ("fp32" if a.workload == "sphere512" else "fp64")
     cfg = mk(steps=a.steps + a.warmup, precision=prec, **kw)
     cfg.conv_history = 2
+    cfg.kahan = 0 if a.no_kahan else 1
     nodes = cfg.NI * cfg.NJ * cfg.NK
     s = nsfsi.Solver(cfg)
     s.initialize()
@@ -106,7 +109,8 @@ def main():
     else:
         tile = s.autotune(2)
     s.setOption("graph", 0 if a.no_graph else 1)
-    s.setOption("fused_scans", 1 if a.fused_scans else 0)
+    if a.fused_scans is not None:
+        s.setOption("fused_scans", a.fused_scans)
     s.marchTimeStep(a.warmup)
     t0 = time.perf_counter()
     s.marchTimeStep(a.steps)  # returns after a device synchronisation
@@ -134,9 +138,11 @@ def main():
         "dtype": "f32" if prec == "fp32" else "f64",
         "data": "synthetic (seeded freestream + perturbation initial field)",
         "config": {"workload": a.workload, "grid": [cfg.NI, cfg.NJ, cfg.NK], "precision": prec,
+                   "compensated_combine": bool(prec == "fp32" and cfg.kahan),
                    "ghosts": g, "filter_nodes": f, "stage_kernel": a.kernel,
                    "tile_rows_x_zplanes": list(tile) if tile else None,
-                   "hip_graph": not a.no_graph},
+                   "hip_graph": not a.no_graph,
+                   "fused_scans": bool(a.kernel == "fused" and (a.fused_scans is None or a.fused_scans))},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1) if achieved else None,

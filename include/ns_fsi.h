@@ -178,9 +178,9 @@ ns_status ns_profile(ns_handle* h, int32_t nsteps, double ms[4], int32_t launche
  *   "tile_kc"      z-planes per block of the fused stage (default 64);
  *   "autotune"     after ns_init: time every (tile_ty, tile_kc) pair on this case with `value` launches
  *                  each (the state is not changed) and keep the fastest;
- *   "fused_scans"  1: the end-of-step scans (norms, dt maxima, finite check) run inside the last stage
- *                  kernel plus a small boundary pass; 0 (default): one separate pass over the grid (faster
- *                  on MI355X: the scan raises the stage kernel's register use and lowers its occupancy);
+ *   "fused_scans"  1 (default): the end-of-step scans (norms, dt maxima, finite check) run inside the
+ *                  last fused stage kernel plus a small boundary pass; 0: one separate pass over the grid
+ *                  (the unfused stage kernel always uses the separate pass);
  *   "graph"        1: ns_step replays each time step as one captured HIP graph (no per-kernel launch
  *                  cost), 0 (default): launches kernel by kernel. Same results either way.
  * Results do not depend on the tile shape (the per-node arithmetic is the same). */

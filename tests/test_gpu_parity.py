@@ -124,6 +124,26 @@ def test_trajectory_fp64(name, steps, kernel):
         assert np.allclose(hist[name_], oh, rtol=1e-10, atol=0)
 
 
+@pytest.mark.parametrize("name,steps", [("sphere24", 20), ("cylinder_z64", 10)])
+def test_trajectory_fp64_separate_scans(name, steps):
+    """End-of-step scans as one separate pass over the grid instead of inside the last stage kernel:
+    same dt, time, norm history and state as the oracle."""
+    cfg = CLASSIFY_CASES[name]()
+    s, o = _pair(cfg)
+    s.setOption("fused_scans", 0)
+    s.marchTimeStep(steps)
+    o.march(steps)
+    t, lv, dt = s.time()
+    ot, olv, odt = o.state()
+    assert lv == olv == steps
+    assert abs(t - ot) <= 1e-12 * abs(ot)
+    assert abs(dt - odt) <= 1e-12 * abs(odt)
+    _traj_check(cfg, steps, s, o)
+    hist = s.getConvergenceHistory()
+    for q, name_ in enumerate(nsfsi.FIELDS[:5]):
+        assert np.allclose(hist[name_], o.norm_history(q), rtol=1e-10, atol=0)
+
+
 def test_geometry_error_matches_oracle():
     """At 40^3 the z-cylinder's ghosts on the top symmetry plane get an image-point box that leaves the
     mesh (getSurroundingNodesBox clamps to N, src/SmallVectors.h:120-122): both paths must refuse."""
