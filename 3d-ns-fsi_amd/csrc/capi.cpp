@@ -178,6 +178,17 @@ ns_status ns_norm_history(ns_handle* h, int32_t var, double* out, int64_t n, int
     });
 }
 
+ns_status ns_force_history(ns_handle* h, double* out, int64_t n, int64_t* count) {
+    return guarded(h, [&] {
+        auto& c = live(h);
+        c.readScalars();
+        const int64_t avail = (int64_t)(c.forceHost.size() / 4);
+        if (count) *count = avail;
+        for (int64_t q = 0; out && q < n && q < avail; ++q)
+            for (int v = 0; v < 4; ++v) out[4 * q + v] = c.forceHost[4 * q + v];
+    });
+}
+
 ns_status ns_totals(ns_handle* h, double out[5]) {
     return guarded(h, [&] {
         if (!out) throw std::invalid_argument("null output");

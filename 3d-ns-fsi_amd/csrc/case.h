@@ -94,6 +94,10 @@ public:
     double* hist = nullptr;
     int histCap = 1024;
     std::vector<double> normsHost[5];
+    // force on the body after every step while the "body_force" option is on: (timeLevel, Fx, Fy, Fz)
+    bool bodyForce = false;
+    double* forceRing = nullptr;
+    std::vector<double> forceHost;
     double t = 0, dt = 0;
     uint64_t level = 0;
     bool nonFinite = false;

@@ -320,6 +320,7 @@ void Case::allocFields() {
     partials = (double*)alloc(slots * reducePartialSlots() * sizeof(double));
     scal = (StepScalars*)alloc(sizeof(StepScalars));
     hist = (double*)alloc((size_t)histCap * 5 * sizeof(double));
+    forceRing = (double*)alloc((size_t)histCap * 4 * sizeof(double));
     StepScalars s0{};
     hipCheck(hipMemcpyAsync(scal, &s0, sizeof(s0), hipMemcpyHostToDevice, stream), "copy scalars");
 }
@@ -492,6 +493,9 @@ void Case::stepOnce() {
     } else {
         computeDt<T>(1);
     }
+    if (bodyForce && nGhost > 0)  // the scan partials are consumed: reuse them
+        launchBodyForce<T>(stream, L, P, W, (const T*)A, cls, (const T*)rhoPre, (const T*)rhoEPre, ghostList, nGhost,
+                           partials, scal, forceRing, histCap);
     mark(3, false);
     std::swap(U, A);
 }
@@ -503,6 +507,8 @@ void Case::setOption(const std::string& name, int64_t value) {
         stageKernel = (int)value;
     } else if (name == "fused_scans") {
         fusedScans = value != 0;
+    } else if (name == "body_force") {
+        bodyForce = value != 0;
     } else if (name == "graph") {
         useGraph = value != 0;
     } else if (name == "tile_ty") {
@@ -659,6 +665,15 @@ void Case::readScalars() {
         for (uint64_t q = have; q < s.histCount; ++q) {
             int slot = (int)(q % (uint64_t)histCap);
             for (int v = 0; v < 5; ++v) normsHost[v].push_back(h[5 * slot + v]);
+        }
+    }
+    uint64_t haveF = forceHost.size() / 4;
+    if (s.forceCount > haveF) {
+        std::vector<double> h((size_t)histCap * 4);
+        hipCheck(hipMemcpy(h.data(), forceRing, h.size() * sizeof(double), hipMemcpyDeviceToHost), "copy forces");
+        for (uint64_t q = haveF; q < s.forceCount; ++q) {
+            int slot = (int)(q % (uint64_t)histCap);
+            for (int v = 0; v < 4; ++v) forceHost.push_back(h[4 * slot + v]);
         }
     }
 }

@@ -18,6 +18,7 @@ struct StepScalars {
     double dt;
     uint64_t level;
     uint64_t histCount;
+    uint64_t forceCount;  // entries written to the body-force ring
     int32_t nonFinite;
     int32_t filterNow;  // periodic density filter active in the current step
 };
@@ -69,6 +70,14 @@ int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P,
                      const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
                      const int32_t* filterFlag, uint16_t* comp);
 const int* fusedTileHeights(int* n);
+// Force on the body from the state S after a step's boundary application (FP64; the discrete momentum
+// balance over the faces between fluid interior and ghost nodes): appends (timeLevel, Fx, Fy, Fz) to
+// `ring` (cap entries of 4) and advances st->forceCount. partials: bodyForceBlocks(nGhost) x 3 doubles.
+int bodyForceBlocks(int64_t nGhost);
+template <class T>
+void launchBodyForce(hipStream_t s, const Layout& L, const Params& P, const Window& W, const T* S, const uint8_t* cls,
+                     const T* rhoPre, const T* rhoEPre, const int64_t* ghosts, int64_t nGhost, double* partials,
+                     StepScalars* st, double* ring, int cap);
 int fusedMaxBlocks(const Layout& L);  // largest block count over all tile shapes
 
 // End-of-step scans over what the stage-4 pass leaves out, after the boundary conditions: the owned

@@ -47,6 +47,28 @@ struct Coef {
 };
 
 template <class T>
+Coef<T> makeCoef(const Params& P) {
+    Coef<T> c;
+    const double dx = P.hx, dy = P.hy, dz = P.hz;
+    c.i2dx = (T)(1 / (2 * dx));
+    c.i2dy = (T)(1 / (2 * dy));
+    c.i2dz = (T)(1 / (2 * dz));
+    c.c43x = (T)(2 / (3 * dx * dx));
+    c.c43y = (T)(2 / (3 * dy * dy));
+    c.c43z = (T)(2 / (3 * dz * dz));
+    c.c1x = (T)(1 / (2 * dx * dx));
+    c.c1y = (T)(1 / (2 * dy * dy));
+    c.c1z = (T)(1 / (2 * dz * dz));
+    c.gamma = (T)P.gamma;
+    c.gm1 = (T)(P.gamma - 1);
+    c.H0 = (T)(1 / (P.gamma - 1));
+    c.ScP1 = (T)P.ScPlusOne;
+    c.Re = (T)P.Re;
+    c.pf = (T)(1 / ((P.gamma - 1) * P.Pr));
+    return c;
+}
+
+template <class T>
 struct FusedArgs {
     Layout L;
     Window W;
@@ -483,6 +505,139 @@ void k_stage_fused(FusedArgs<T> a) {
     }
 }
 
+// ---- force on the body: the discrete momentum balance of the flux form. Summed over all fluid interior
+// nodes, the right-hand sides telescope to the face fluxes across the outer boundary and across the faces
+// between interior nodes and ghost nodes; the latter are the momentum per unit time the body gives the
+// fluid. The force on the body is minus that sum times the node volume. Evaluated in FP64 from the state
+// after the step's last boundary application, with the stage kernel's own face-flux functions.
+template <class T>
+struct ForceArgs {
+    Layout L;
+    Window W;
+    Coef<double> c;
+    const T* S;
+    const uint8_t* cls;
+    const T* rhoPre;
+    const T* rhoEPre;
+};
+
+template <class T>
+__device__ void primsAt(const ForceArgs<T>& a, int i, int j, int k, Pt<double>& q) {
+    const Layout& L = a.L;
+    i = min(max(i, 0), L.NI - 1);
+    j = min(max(j, 0), L.NJ - 1);
+    k = min(max(k, 0), L.NK - 1);
+    const size_t o = L.off(i, j, k);
+    q.rho = (double)a.S[o];
+    q.ru = (double)a.S[o + L.fstride];
+    q.rv = (double)a.S[o + 2 * L.fstride];
+    q.rw = (double)a.S[o + 3 * L.fstride];
+    q.E = (double)a.S[o + 4 * L.fstride];
+    double rp = q.rho, Ep = q.E;
+    if (a.W.contains(i, j, k) && (a.cls[o] & CLS_FILTER)) {  // primitives stay pre-filter there
+        const size_t w = a.W.idx(i, j, k);
+        rp = (double)a.rhoPre[w];
+        Ep = (double)a.rhoEPre[w];
+    }
+    prims(a.c, rp, q.ru, q.rv, q.rw, Ep, q);
+}
+
+// Point with all nine velocity derivatives (central differences; dz* returned separately).
+template <class T>
+__device__ void pointAt(const ForceArgs<T>& a, int i, int j, int k, Pt<double>& q, double dz[3]) {
+    primsAt(a, i, j, k, q);
+    Pt<double> m, p;
+    primsAt(a, i - 1, j, k, m);
+    primsAt(a, i + 1, j, k, p);
+    q.dxu = (p.u - m.u) * a.c.i2dx;
+    q.dxv = (p.v - m.v) * a.c.i2dx;
+    q.dxw = (p.w - m.w) * a.c.i2dx;
+    primsAt(a, i, j - 1, k, m);
+    primsAt(a, i, j + 1, k, p);
+    q.dyu = (p.u - m.u) * a.c.i2dy;
+    q.dyv = (p.v - m.v) * a.c.i2dy;
+    q.dyw = (p.w - m.w) * a.c.i2dy;
+    primsAt(a, i, j, k - 1, m);
+    primsAt(a, i, j, k + 1, p);
+    dz[0] = (p.u - m.u) * a.c.i2dz;
+    dz[1] = (p.v - m.v) * a.c.i2dz;
+    dz[2] = (p.w - m.w) * a.c.i2dz;
+}
+
+template <class T>
+__device__ void nodalFlux(const ForceArgs<T>& a, int d, int i, int j, int k, double g[5], Pt<double>& q) {
+    double dz[3];
+    pointAt(a, i, j, k, q, dz);
+    if (d == 0) gx(a.c, q, dz[0], dz[2], g);
+    else if (d == 1) gy(a.c, q, dz[1], dz[2], g);
+    else gz(a.c, q, g);
+}
+
+// One block partial (3 force components on the fluid, before the volume factor) per block.
+template <class T>
+__global__ void k_body_force(ForceArgs<T> a, const int64_t* ghosts, int64_t n, double* partials) {
+    double f[3] = {0, 0, 0};
+    const Layout& L = a.L;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t id = ghosts[q];
+        const int gi = (int)(id % L.NI), gj = (int)((id / L.NI) % L.NJ), gk = (int)(id / ((int64_t)L.NI * L.NJ));
+        for (int d = 0; d < 3; ++d)
+            for (int s = -1; s <= 1; s += 2) {
+                const int pi = gi + (d == 0 ? s : 0), pj = gj + (d == 1 ? s : 0), pk = gk + (d == 2 ? s : 0);
+                if (pi < 0 || pi >= L.NI || pj < 0 || pj >= L.NJ || pk < 0 || pk >= L.NK) continue;
+                if ((a.cls[L.off(pi, pj, pk)] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) continue;
+                // face between the lower node lo and the upper node up along d
+                const bool pUp = s > 0;
+                double glo[5], gup[5], cf[5];
+                Pt<double> lo, up;
+                nodalFlux(a, d, pUp ? gi : pi, pUp ? gj : pj, pUp ? gk : pk, glo, lo);
+                nodalFlux(a, d, pUp ? pi : gi, pUp ? pj : gj, pUp ? pk : gk, gup, up);
+                const double i2d = d == 0 ? a.c.i2dx : (d == 1 ? a.c.i2dy : a.c.i2dz);
+                if (d == 0) compact<double, 0>(a.c, lo, up, cf);
+                else if (d == 1) compact<double, 1>(a.c, lo, up, cf);
+                else compact<double, 2>(a.c, lo, up, cf);
+                // the face flux enters RHS(lower) with +, RHS(upper) with -
+                const double sg = pUp ? -1.0 : 1.0;
+                for (int c = 0; c < 3; ++c) f[c] += sg * ((glo[1 + c] + gup[1 + c]) * i2d + cf[1 + c]);
+            }
+    }
+    __shared__ double red[3][32];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c = 0; c < 3; ++c) {
+        double v = wave_sum(f[c]);
+        if (lane == 0) red[c][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double v = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[threadIdx.x][w];
+        partials[(size_t)blockIdx.x * 3 + threadIdx.x] = v;
+    }
+}
+
+__global__ void k_force_finalize(const double* partials, int nb, double vol, StepScalars* st, double* ring, int cap) {
+    __shared__ double red[3][4];
+    double f[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int c = 0; c < 3; ++c) f[c] += partials[(size_t)b * 3 + c];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c = 0; c < 3; ++c) {
+        double v = wave_sum(f[c]);
+        if (lane == 0) red[c][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int slot = (int)(st->forceCount % (uint64_t)cap);
+        ring[4 * slot] = (double)st->level;
+        for (int c = 0; c < 3; ++c) {
+            double v = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[c][w];
+            ring[4 * slot + 1 + c] = -vol * v;  // force on the body
+        }
+        st->forceCount += 1;
+    }
+}
+
 template <class T, int TY>
 int fusedBlocks(const Layout& L, int kc) {
     return ((L.NI + 61) / 62) * ((L.NJ + TY - 1) / TY) * ((L.NK + kc - 1) / kc);
@@ -536,22 +691,7 @@ int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P,
     a.vm = (T)std::max(4. / 3, P.gamma / P.Pr);
     a.L = L;
     a.W = W;
-    const double dx = P.hx, dy = P.hy, dz = P.hz;
-    a.c.i2dx = (T)(1 / (2 * dx));
-    a.c.i2dy = (T)(1 / (2 * dy));
-    a.c.i2dz = (T)(1 / (2 * dz));
-    a.c.c43x = (T)(2 / (3 * dx * dx));
-    a.c.c43y = (T)(2 / (3 * dy * dy));
-    a.c.c43z = (T)(2 / (3 * dz * dz));
-    a.c.c1x = (T)(1 / (2 * dx * dx));
-    a.c.c1y = (T)(1 / (2 * dy * dy));
-    a.c.c1z = (T)(1 / (2 * dz * dz));
-    a.c.gamma = (T)P.gamma;
-    a.c.gm1 = (T)(P.gamma - 1);
-    a.c.H0 = (T)(1 / (P.gamma - 1));
-    a.c.ScP1 = (T)P.ScPlusOne;
-    a.c.Re = (T)P.Re;
-    a.c.pf = (T)(1 / ((P.gamma - 1) * P.Pr));
+    a.c = makeCoef<T>(P);
     if (L.fstride * sizeof(T) > 0xFFFFFFFFull) throw std::length_error("field larger than 4 GiB: 32-bit offsets");
     for (int q = 0; q < 5; ++q) {
         a.Sin[q] = Sin + q * L.fstride;
@@ -574,6 +714,32 @@ int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P,
 }
 
 int fusedMaxBlocks(const Layout& L) { return fusedBlocks<float, 4>(L, 16); }
+
+int bodyForceBlocks(int64_t nGhost) { return (int)std::min<int64_t>(256, std::max<int64_t>(1, (nGhost + 255) / 256)); }
+
+template <class T>
+void launchBodyForce(hipStream_t s, const Layout& L, const Params& P, const Window& W, const T* S, const uint8_t* cls,
+                     const T* rhoPre, const T* rhoEPre, const int64_t* ghosts, int64_t nGhost, double* partials,
+                     StepScalars* st, double* ring, int cap) {
+    ForceArgs<T> a;
+    a.L = L;
+    a.W = W;
+    a.c = makeCoef<double>(P);
+    a.S = S;
+    a.cls = cls;
+    a.rhoPre = rhoPre;
+    a.rhoEPre = rhoEPre;
+    const int nb = bodyForceBlocks(nGhost);
+    hipLaunchKernelGGL(k_body_force<T>, dim3(nb), dim3(256), 0, s, a, ghosts, nGhost, partials);
+    hipLaunchKernelGGL(k_force_finalize, dim3(1), dim3(256), 0, s, partials, nb, P.hx * P.hy * P.hz, st, ring, cap);
+}
+
+template void launchBodyForce<double>(hipStream_t, const Layout&, const Params&, const Window&, const double*,
+                                      const uint8_t*, const double*, const double*, const int64_t*, int64_t, double*,
+                                      StepScalars*, double*, int);
+template void launchBodyForce<float>(hipStream_t, const Layout&, const Params&, const Window&, const float*,
+                                     const uint8_t*, const float*, const float*, const int64_t*, int64_t, double*,
+                                     StepScalars*, double*, int);
 
 const int* fusedTileHeights(int* n) {
     static const int v[4] = {4, 6, 8, 14};

@@ -104,6 +104,7 @@ def lib():
     L.ns_get_filter_nodes.argtypes = [H, C.POINTER(C.c_int32)]
     L.ns_norm_history.argtypes = [H, C.c_int32, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]
     L.ns_totals.argtypes = [H, C.POINTER(C.c_double)]
+    L.ns_force_history.argtypes = [H, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]
     L.ns_get_geometry.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.ns_profile.argtypes = [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.ns_set_option.argtypes = [H, C.c_char_p, C.c_int64]
@@ -115,7 +116,7 @@ def lib():
     L.ns_abi_version.restype = C.c_int32
     for name in ("ns_config_parse", "ns_create", "ns_init", "ns_set_initial_field", "ns_step", "ns_run", "ns_get_time",
                  "ns_get_field", "ns_get_node_types", "ns_get_counts", "ns_get_ghosts", "ns_get_filter_nodes",
-                 "ns_norm_history", "ns_totals", "ns_get_geometry", "ns_profile", "ns_set_option", "ns_get_tile"):
+                 "ns_norm_history", "ns_force_history", "ns_totals", "ns_get_geometry", "ns_profile", "ns_set_option", "ns_get_tile"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -348,6 +349,33 @@ class Solver:
                 self._check(lib().ns_norm_history(self._h, q, _dptr(a), cnt.value, C.byref(cnt)))
             out[name] = a
         return out
+
+    def getForceHistory(self) -> np.ndarray:
+        """(n, 4) array of (timeLevel, Fx, Fy, Fz): force on the body after each step taken with the
+        "body_force" option on (discrete momentum balance over the fluid/ghost faces, FP64)."""
+        cnt = C.c_int64()
+        self._check(lib().ns_force_history(self._h, None, 0, C.byref(cnt)))
+        a = np.zeros((cnt.value, 4))
+        if cnt.value:
+            self._check(lib().ns_force_history(self._h, _dptr(a), cnt.value, C.byref(cnt)))
+        return a
+
+    def referenceArea(self) -> float:
+        """Frontal area for force coefficients: pi r^2 (sphere), 2 r times the span of FluidInterior planes
+        along the axis (cylinder)."""
+        c = self.config
+        if c.body_type == BODY["sphere"]:
+            return float(np.pi * c.body_radius ** 2)
+        if c.body_type == BODY["cylinder"]:
+            n = (c.NI, c.NJ, c.NK)[c.body_axis]
+            h = (self.mesh.dx, self.mesh.dy, self.mesh.dz)[c.body_axis]
+            return float(2 * c.body_radius * (n - 2) * h)
+        raise ValueError("no body")
+
+    def dragCoefficient(self) -> np.ndarray:
+        """C_D = Fx / (0.5 rho_0 M_0^2 A_ref) per recorded step (velocities are in units of c_0)."""
+        f = self.getForceHistory()
+        return f[:, 1] / (0.5 * self.config.M_0 ** 2 * self.referenceArea())
 
     def close(self):
         if self._h:

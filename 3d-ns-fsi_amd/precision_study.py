@@ -1,0 +1,105 @@
+"""Precision study: FP32 forms of the time step against the FP64 form on one case.
+
+    python 3d-ns-fsi_amd/precision_study.py [--workload cylinder256] [--N 256] [--steps 1000] [--out FILE]
+
+Runs the case in FP64 (the form that matches the serial FP64 oracle to <= 1e-12 relative L2 per field,
+see tests/test_gpu_parity.py), in FP32 with the compensated combine and in FP32 with the plain combine,
+each with the body-force record on, and reports for each FP32 form the relative L2 and Linf error of every
+conserved field against FP64 after the last step, and the drag coefficient (last step and mean over the
+last tenth of the run) of every form. The FP64 device run is the reference at these sizes because the
+serial oracle would need hours for 256^3 x 1000 steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+import cases  # noqa: E402
+import nsfsi  # noqa: E402
+
+
+def run(cfg, steps, label):
+    t0 = time.perf_counter()
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    s.setOption("body_force", 1)
+    s.setOption("graph", 1)
+    done = 0
+    while done < steps:  # progress lines for long runs
+        n = min(250, steps - done)
+        s.marchTimeStep(n)
+        done += n
+        print(f"  {label}: {done}/{steps} steps", flush=True)
+    U = s.mesh.conserved()
+    cd = s.dragCoefficient()
+    t, lv, dt = s.time()
+    s.close()
+    return U, cd, t, time.perf_counter() - t0
+
+
+def errors(U, V):
+    out = []
+    for q in range(5):
+        d = U[q] - V[q]
+        n2 = np.linalg.norm(V[q])
+        ni = np.abs(V[q]).max()
+        out.append((float(np.linalg.norm(d) / n2) if n2 > 0 else float(np.linalg.norm(d)),
+                    float(np.abs(d).max() / ni) if ni > 0 else float(np.abs(d).max())))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="cylinder256", choices=sorted(cases.WORKLOADS))
+    ap.add_argument("--N", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--out", default=None, help="also write the report (markdown) here")
+    ap.add_argument("--json", default=None, help="also write the figures as JSON here")
+    a = ap.parse_args()
+    kw = {"N": a.N} if a.N else {}
+    forms = [("fp64", 0), ("fp32", 1), ("fp32", 0)]
+    res = {}
+    for prec, kahan in forms:
+        cfg = cases.WORKLOADS[a.workload](steps=a.steps, precision=prec, **kw)
+        cfg.kahan = kahan
+        print(f"run {a.workload} {prec} kahan={kahan} grid {cfg.NI}x{cfg.NJ}x{cfg.NK}", flush=True)
+        res[(prec, kahan)] = run(cfg, a.steps, f"{prec} kahan={kahan}")
+    ref = res[("fp64", 0)]
+    last = max(1, a.steps // 10)
+    lines = [f"# Precision study: {a.workload}, grid {cfg.NI}x{cfg.NJ}x{cfg.NK}, {a.steps} RK4 steps", "",
+             "Errors are relative to the FP64 device run (relative L2 = ||U - U64|| / ||U64||, relative Linf =",
+             "max|U - U64| / max|U64|, per conserved field, after the last step). C_D = Fx / (0.5 M_0^2 A_ref).", "",
+             "| form | t_end | C_D last | C_D mean (last tenth) | wall s |", "|---|---|---|---|---|"]
+    names = {("fp64", 0): "FP64", ("fp32", 1): "FP32 + compensated combine", ("fp32", 0): "FP32 plain combine"}
+    for k in forms:
+        U, cd, t, wall = res[k]
+        lines.append(f"| {names[k]} | {t:.6f} | {cd[-1]:.8f} | {cd[-last:].mean():.8f} | {wall:.1f} |")
+    lines += ["", "| form | field | rel L2 | rel Linf |", "|---|---|---|---|"]
+    report = {"workload": a.workload, "grid": [cfg.NI, cfg.NJ, cfg.NK], "steps": a.steps, "forms": {}}
+    for k in forms[1:]:
+        errs = errors(res[k][0], ref[0])
+        for q, (l2, li) in enumerate(errs):
+            lines.append(f"| {names[k]} | {nsfsi.FIELDS[q]} | {l2:.3e} | {li:.3e} |")
+        cdr = abs(res[k][1][-1] - ref[1][-1]) / abs(ref[1][-1])
+        lines.append(f"| {names[k]} | C_D (last step) | {cdr:.3e} | |")
+        report["forms"][names[k]] = {"rel_l2": [e[0] for e in errs], "rel_linf": [e[1] for e in errs],
+                                     "cd_rel_err": cdr}
+    for k in forms:
+        report["forms"].setdefault(names[k], {})["cd_last"] = float(res[k][1][-1])
+    text = "\n".join(lines) + "\n"
+    print(text)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(text)
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(report, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

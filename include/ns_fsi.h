@@ -158,6 +158,19 @@ ns_status ns_get_filter_nodes(ns_handle* h, int32_t* ijk);
  * var 0..4 = rho..rho_E; copies min(n, available) values, *count receives the number available. */
 ns_status ns_norm_history(ns_handle* h, int32_t var, double* out, int64_t n, int64_t* count);
 
+/* Force on the immersed body after each step taken while the run-time option "body_force" is 1 (no
+ * reference counterpart: the reference computes no forces). Definition: the discrete momentum balance of
+ * the time step. Summed over all FluidInterior nodes, the RK4 right-hand side of the momentum equations
+ * telescopes into face fluxes across the outer boundary and across the faces between FluidInterior and
+ * ghost nodes; the latter are the momentum per unit time the body gives the fluid. The force on the body
+ * is minus that sum times dx*dy*dz, evaluated in FP64 on the state after the step's last boundary
+ * application, in the nondimensional units of the state (density rho_0, velocity c_0, length 1).
+ * Drag coefficient: C_D = Fx / (0.5 * M_0^2 * A_ref) with A_ref = pi r^2 (sphere) or 2 r times the span
+ * of FluidInterior planes along the cylinder axis ((N_axis - 2) * h_axis).
+ * out receives min(n, available) entries of 4 doubles (timeLevel after the step, Fx, Fy, Fz); *count
+ * the number available. */
+ns_status ns_force_history(ns_handle* h, double* out, int64_t n, int64_t* count);
+
 /* FP64 totals over FluidInterior nodes of the current state: sum of each conserved variable times
  * dx*dy*dz (mass, momenta, energy). No reference counterpart; conservation monitor. */
 ns_status ns_totals(ns_handle* h, double out[5]);
@@ -181,6 +194,7 @@ ns_status ns_profile(ns_handle* h, int32_t nsteps, double ms[4], int32_t launche
  *   "fused_scans"  1 (default): the end-of-step scans (norms, dt maxima, finite check) run inside the
  *                  last fused stage kernel plus a small boundary pass; 0: one separate pass over the grid
  *                  (the unfused stage kernel always uses the separate pass);
+ *   "body_force"   1: record the force on the body after every step (ns_force_history), 0 (default);
  *   "graph"        1: ns_step replays each time step as one captured HIP graph (no per-kernel launch
  *                  cost), 0 (default): launches kernel by kernel. Same results either way.
  * Results do not depend on the tile shape (the per-node arithmetic is the same). */
