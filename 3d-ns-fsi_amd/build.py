@@ -25,6 +25,7 @@ SOURCES = [
     ("stage_plain.hip", False),
     ("stage_fused.hip", False),
     ("reduce.hip", False),
+    ("batch.cpp", False),
 ]
 
 ARCH = "gfx950"

@@ -8,6 +8,8 @@ convStabilityLimit; viscStabilityLimit stays at the ConfigFile's -0.68; the peri
 """
 from __future__ import annotations
 
+import numpy as np
+
 import nsfsi
 
 SIDE_WALLS = ["inlet", "outlet", "symmetry", "symmetry", "symmetry", "symmetry"]
@@ -50,6 +52,23 @@ def sphere96(N=96, steps=400, precision="fp64"):
 def reference_default():
     """The reference's shipped ConfigFile: 161x83x3 quasi-2-D cylinder channel, periodic z (ConfigFile:22-125)."""
     return nsfsi.make_config()
+
+
+def sweep64(n=64, steps=500, seed=4, precision="fp32"):
+    """Batched sweep: n independent cases; case c draws from a generator seeded with seed + c: grid edge
+    N in {32, 48, 64, 96, 128}, sphere radius U[0.3, 0.8] at the origin of [-2,2]^3, Mach U[0.1, 0.4],
+    Re_D U[50, 300] (Re = Re_D / (M * 2r)); Gaussian perturbation sigma 1e-3 seeded with seed + c."""
+    out = []
+    for c in range(n):
+        rng = np.random.default_rng(seed + c)
+        N = int(rng.choice([32, 48, 64, 96, 128]))
+        r = float(rng.uniform(0.3, 0.8))
+        M = float(rng.uniform(0.1, 0.4))
+        Re_D = float(rng.uniform(50.0, 300.0))
+        out.append(_base(N, 4.0, Re_D, M, 2 * r, steps, body_type="sphere", body_centre=(2.0, 2.0, 2.0),
+                         body_radius=r, ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=seed + c,
+                         precision=precision))
+    return out
 
 
 WORKLOADS = {

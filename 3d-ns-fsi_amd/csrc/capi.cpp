@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/ns_fsi.h"
+#include "batch_host.h"
 #include "case.h"
 
 struct ns_handle {
@@ -14,10 +15,15 @@ struct ns_handle {
     std::string err;
 };
 
+struct ns_batch {
+    std::unique_ptr<nsfsi::Batch> b;
+    std::string err;
+};
+
 namespace {
 
-template <class F>
-ns_status guarded(ns_handle* h, F f) {
+template <class H, class F>
+ns_status guarded(H* h, F f) {
     if (!h) return NS_ERR_ARGUMENT;
     try {
         f();
@@ -49,6 +55,17 @@ ns_status guarded(ns_handle* h, F f) {
 nsfsi::Case& live(ns_handle* h) {
     if (!h->c) throw nsfsi::StateError("handle has no case");
     return *h->c;
+}
+
+nsfsi::Batch& live(ns_batch* h) {
+    if (!h->b) throw nsfsi::StateError("handle has no batch");
+    return *h->b;
+}
+
+nsfsi::Case& batchCase(ns_batch* h, int32_t c) {
+    auto& b = live(h);
+    if (c < 0 || c >= (int32_t)b.cases.size()) throw std::invalid_argument("case index out of range");
+    return *b.cases[c];
 }
 
 }  // namespace
@@ -237,5 +254,81 @@ const char* ns_last_error(ns_handle* h) {
 }
 
 void ns_destroy(ns_handle* h) { delete h; }
+
+ns_status ns_batch_create(const ns_config* cfgs, int32_t n, ns_batch** out) {
+    if (!cfgs || !out || n < 1) return NS_ERR_ARGUMENT;
+    auto* h = new ns_batch();
+    *out = h;
+    return guarded(h, [&] { h->b = std::make_unique<nsfsi::Batch>(cfgs, n); });
+}
+
+ns_status ns_batch_init(ns_batch* h) {
+    return guarded(h, [&] { live(h).init(); });
+}
+
+ns_status ns_batch_step(ns_batch* h, int32_t nsteps) {
+    ns_status s = guarded(h, [&] {
+        if (nsteps < 0) throw std::invalid_argument("negative step count");
+        live(h).step(nsteps);
+    });
+    if (s == NS_OK)
+        for (size_t c = 0; c < h->b->cases.size(); ++c)
+            if (h->b->cases[c]->nonFinite) {
+                h->err = "Non-finite value found in the conserved variables of case " + std::to_string(c);
+                return NS_ERR_NONFINITE;
+            }
+    return s;
+}
+
+ns_status ns_batch_get_time(ns_batch* h, int32_t c, double* t, uint64_t* level, double* dt) {
+    return guarded(h, [&] {
+        auto& k = batchCase(h, c);
+        if (t) *t = k.t;
+        if (level) *level = k.level;
+        if (dt) *dt = k.dt;
+    });
+}
+
+ns_status ns_batch_get_field(ns_batch* h, int32_t c, int32_t which, double* out) {
+    return guarded(h, [&] {
+        if (!out || which < 0 || which > 11) throw std::invalid_argument("bad field id or null output");
+        batchCase(h, c).getField(which, out);
+    });
+}
+
+ns_status ns_batch_norm_history(ns_batch* h, int32_t c, int32_t var, double* out, int64_t n, int64_t* count) {
+    return guarded(h, [&] {
+        if (var < 0 || var > 4) throw std::invalid_argument("var must be 0..4");
+        auto& k = batchCase(h, c);
+        const auto& v = k.normsHost[var];
+        int64_t avail = (int64_t)v.size();
+        bool stored = k.cfg.conv_history == 2 || (k.cfg.conv_history == 1 && var == 0);
+        if (!stored) avail = 0;
+        if (count) *count = avail;
+        for (int64_t q = 0; out && q < n && q < avail; ++q) out[q] = v[q];
+    });
+}
+
+ns_status ns_batch_set_option(ns_batch* h, const char* name, int64_t value) {
+    return guarded(h, [&] {
+        if (!name) throw std::invalid_argument("null option name");
+        live(h).setOption(name, value);
+    });
+}
+
+ns_status ns_batch_get_tile(ns_batch* h, int32_t* ty, int32_t* kc) {
+    return guarded(h, [&] {
+        auto& b = live(h);
+        if (ty) *ty = b.tile.ty;
+        if (kc) *kc = b.tile.kc;
+    });
+}
+
+const char* ns_batch_last_error(ns_batch* h) {
+    if (!h) return "null handle";
+    return h->err.c_str();
+}
+
+void ns_batch_destroy(ns_batch* h) { delete h; }
 
 }  // extern "C"

@@ -90,6 +90,7 @@ public:
     StepScalars* scal = nullptr;
     double* partials = nullptr;
     int nPartials = 0;
+    size_t partialSlots = 0;  // capacity of `partials` in slots of reducePartialSlots() doubles
     int nStage4Partials = 0;
     double* hist = nullptr;
     int histCap = 1024;

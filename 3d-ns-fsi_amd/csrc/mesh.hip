@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "device_common.h"
+#include "batch.h"
 #include "kernels.h"
 
 namespace nsfsi {
@@ -304,10 +305,11 @@ __device__ __forceinline__ Fields<T> fields(const Layout& L, T* S) {
 // One edge face, all owned nodes (src/Boundary.cpp:250-417). Each thread handles one node of the face;
 // the inlet's in-plane density filter recomputes its neighbours' extrapolated densities itself, so the
 // face needs no second pass.
+// Node (a, b) of the face's owned box (a along the first in-plane axis, b along the second); ts is the
+// time of the stage the boundary values belong to.
 template <class T>
-__global__ void k_edge_face(Layout L, Params P, EdgeFace f, T* S, const double* tdev, const double* dtdev, double frac) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;  // first in-plane axis
-    int b = blockIdx.y;                             // second in-plane axis
+__device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, const EdgeFace& f, T* S, double ts, int a,
+                                             int b) {
     // in-plane axes in the reference's loop order: x-normal (j,k), y-normal (i,k), z-normal (i,j)
     int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
     int lo[3] = {f.i0, f.j0, f.k0}, hi[3] = {f.i1, f.j1, f.k1};
@@ -333,11 +335,11 @@ __global__ void k_edge_face(Layout L, Params P, EdgeFace f, T* S, const double* 
         }
         acc += (T)nb * extrap(o);
         acc /= (T)(2 * nb);
-        double ts = *tdev + frac * *dtdev;
         T vel = (T)(fmin(1., ts / P.rampTime) * P.M_0);
         if (f.side == 1) vel *= (T)-1;
-        T uu = 0, vv = 0, ww = 0;
-        if (f.axis == 0) uu = vel; else if (f.axis == 1) vv = vel; else ww = vel;
+        // component selection by selects, not a three-way branch: gfx950 code generation has dropped the
+        // last arm of such a branch on a kernel argument (the z-face case) in this function
+        const T uu = f.axis == 0 ? vel : (T)0, vv = f.axis == 1 ? vel : (T)0, ww = f.axis == 2 ? vel : (T)0;
         T Tt = 0;
         T p = (Tt * ((T)1 + acc) + acc) / e.gamma;
         T rho, ru, rv, rw, rE;
@@ -362,21 +364,43 @@ __global__ void k_edge_face(Layout L, Params P, EdgeFace f, T* S, const double* 
         size_t a2 = o + 2 * s1;
         T u, v, w, p, Tt, mu;
         derivePrims(e, F.f[0][a2], F.f[1][a2], F.f[2][a2], F.f[3][a2], F.f[4][a2], u, v, w, p, Tt, mu);
-        if (f.axis == 0) u = -u; else if (f.axis == 1) v = -v; else w = -w;
+        u = f.axis == 0 ? -u : u;  // selects, see the inlet above
+        v = f.axis == 1 ? -v : v;
+        w = f.axis == 2 ? -w : w;
         T rho, ru, rv, rw, rE;
         deriveCons(e, u, v, w, p, Tt, rho, ru, rv, rw, rE);
         F.f[0][o] = rho; F.f[1][o] = ru; F.f[2][o] = rv; F.f[3][o] = rw; F.f[4][o] = rE;
     }
 }
 
+template <class T>
+__global__ void k_edge_face(Layout L, Params P, EdgeFace f, T* S, const double* tdev, const double* dtdev, double frac) {
+    const double ts = *tdev + frac * *dtdev;
+    edgeFaceNode(L, P, f, S, ts, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+template <class T>
+__global__ void k_edge_face_batched(const CaseView<T>* __restrict__ cv, const int64_t* __restrict__ start, int ncases,
+                                    int face, int role, int parity, double frac) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= start[ncases]) return;
+    const int c = findCase(start, ncases, q);
+    const CaseView<T>& v = cv[c];
+    const EdgeFace& f = v.faces[face];
+    const int ax1 = f.axis == 0 ? 1 : 0;
+    const int lo1 = ax1 == 0 ? f.i0 : f.j0, hi1 = ax1 == 0 ? f.i1 : f.j1;
+    const int64_t r = q - start[c];
+    const int n1 = hi1 - lo1 + 1;
+    const double ts = v.scal->t + frac * v.scal->dt;
+    edgeFaceNode(v.L, v.P, f, roleBuffer(v, role, parity), ts, (int)(r % n1), (int)(r / n1));
+}
+
 // filterClosestFluidNodes, src/Boundary.cpp:618-652, first phase: filtered rho and rho*E into tmp, and
 // the pre-filter values into the side tables (window-dense), which the next stencil evaluation uses for
 // the primitive variables of filter nodes (the reference keeps them pre-filter).
 template <class T>
-__global__ void k_filter_compute(Layout L, Window W, const int64_t* fn, int64_t n, const T* S, T* tmp, T* rhoPre,
-                                 T* rhoEPre) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
+__device__ __forceinline__ void filterComputeNode(const Layout& L, const Window& W, const int64_t* fn, int64_t q, const T* S,
+                                                  T* tmp, T* rhoPre, T* rhoEPre) {
     int i, j, k;
     nodeIJK(L, fn[q], i, j, k);
     size_t o = L.off(i, j, k);
@@ -394,14 +418,46 @@ __global__ void k_filter_compute(Layout L, Window W, const int64_t* fn, int64_t 
 }
 
 template <class T>
-__global__ void k_filter_write(Layout L, const int64_t* fn, int64_t n, T* S, const T* tmp) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
+__device__ __forceinline__ void filterWriteNode(const Layout& L, const int64_t* fn, int64_t q, T* S, const T* tmp) {
     int i, j, k;
     nodeIJK(L, fn[q], i, j, k);
     size_t o = L.off(i, j, k);
     S[o] = tmp[2 * q];
     S[o + 4 * L.fstride] = tmp[2 * q + 1];
+}
+
+template <class T>
+__global__ void k_filter_compute(Layout L, Window W, const int64_t* fn, int64_t n, const T* S, T* tmp, T* rhoPre,
+                                 T* rhoEPre) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) filterComputeNode(L, W, fn, q, S, tmp, rhoPre, rhoEPre);
+}
+
+template <class T>
+__global__ void k_filter_write(Layout L, const int64_t* fn, int64_t n, T* S, const T* tmp) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) filterWriteNode(L, fn, q, S, tmp);
+}
+
+template <class T>
+__global__ void k_filter_compute_batched(const CaseView<T>* __restrict__ cv, const int64_t* __restrict__ start,
+                                         int ncases, int role, int parity) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= start[ncases]) return;
+    const int c = findCase(start, ncases, q);
+    const CaseView<T>& v = cv[c];
+    filterComputeNode(v.L, v.W, v.filterList, q - start[c], (const T*)roleBuffer(v, role, parity), v.ftmp, v.rhoPre,
+                      v.rhoEPre);
+}
+
+template <class T>
+__global__ void k_filter_write_batched(const CaseView<T>* __restrict__ cv, const int64_t* __restrict__ start,
+                                       int ncases, int role, int parity) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= start[ncases]) return;
+    const int c = findCase(start, ncases, q);
+    const CaseView<T>& v = cv[c];
+    filterWriteNode(v.L, v.filterList, q - start[c], roleBuffer(v, role, parity), (const T*)v.ftmp);
 }
 
 __device__ __forceinline__ double luSolveDot(const double* LU, const int8_t* rp, const int8_t* cp, const double* f, double x,
@@ -424,9 +480,7 @@ __device__ __forceinline__ double luSolveDot(const double* LU, const int8_t* rp,
 // values, interpolate (tensor-product lerp when all 8 are fluid, else the stored FP64 Vandermonde
 // factors), mirror into the ghost. Interpolation runs in FP64 whatever the field precision.
 template <class T>
-__global__ void k_ghost_apply(Layout L, Params P, GhostTables G, int64_t n, T* S) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
+__device__ __forceinline__ void ghostApplyNode(const Layout& L, const Params& P, const GhostTables& G, int64_t g, T* S) {
     const uint8_t mask = G.mask[g];
     double val[5][8];
     for (int c = 0; c < 8; ++c) {
@@ -465,6 +519,22 @@ __global__ void k_ghost_apply(Layout L, Params P, GhostTables G, int64_t n, T* S
     S[o + 2 * L.fstride] = (T)(-ipv[2]);
     S[o + 3 * L.fstride] = (T)(-ipv[3]);
     S[o + 4 * L.fstride] = (T)ipv[4];
+}
+
+template <class T>
+__global__ void k_ghost_apply(Layout L, Params P, GhostTables G, int64_t n, T* S) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) ghostApplyNode(L, P, G, g, S);
+}
+
+template <class T>
+__global__ void k_ghost_apply_batched(const CaseView<T>* __restrict__ cv, const int64_t* __restrict__ start, int ncases,
+                                      int role, int parity) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= start[ncases]) return;
+    const int c = findCase(start, ncases, q);
+    const CaseView<T>& v = cv[c];
+    ghostApplyNode(v.L, v.P, v.G, q - start[c], roleBuffer(v, role, parity));
 }
 
 inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -550,7 +620,43 @@ void launchGhostApply(hipStream_t s, const Layout& L, const Params& P, const Gho
     hipLaunchKernelGGL(k_ghost_apply<T>, dim3(blocks(n, 64)), dim3(64), 0, s, L, P, G, n, S);
 }
 
+template <class T>
+void launchEdgeFaceBatched(hipStream_t s, const CaseView<T>* cv, const int64_t* start, int64_t total, int ncases, int face,
+                           int role, int parity, double frac) {
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_edge_face_batched<T>, dim3(blocks(total, 128)), dim3(128), 0, s, cv, start, ncases, face, role,
+                       parity, frac);
+}
+
+template <class T>
+void launchFilterComputeBatched(hipStream_t s, const CaseView<T>* cv, const int64_t* start, int64_t total, int ncases,
+                                int role, int parity) {
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_filter_compute_batched<T>, dim3(blocks(total, 128)), dim3(128), 0, s, cv, start, ncases, role,
+                       parity);
+}
+
+template <class T>
+void launchFilterWriteBatched(hipStream_t s, const CaseView<T>* cv, const int64_t* start, int64_t total, int ncases,
+                              int role, int parity) {
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_filter_write_batched<T>, dim3(blocks(total, 128)), dim3(128), 0, s, cv, start, ncases, role,
+                       parity);
+}
+
+template <class T>
+void launchGhostApplyBatched(hipStream_t s, const CaseView<T>* cv, const int64_t* start, int64_t total, int ncases,
+                             int role, int parity) {
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_ghost_apply_batched<T>, dim3(blocks(total, 64)), dim3(64), 0, s, cv, start, ncases, role, parity);
+}
+
 #define NS_INST(T)                                                                                                           \
+    template void launchEdgeFaceBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int, int,     \
+                                           double);                                                                           \
+    template void launchFilterComputeBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);    \
+    template void launchFilterWriteBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);      \
+    template void launchGhostApplyBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);       \
     template void launchEdgeFace<T>(hipStream_t, const Layout&, const Params&, const EdgeFace&, T*, const double*,            \
                                     const double*, double);                                                                   \
     template void launchFilterCompute<T>(hipStream_t, const Layout&, const Window&, const int64_t*, int64_t, const T*, T*,   \

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 
+#include "batch.h"
 #include "device_common.h"
 #include "stage.h"
 
@@ -102,13 +103,14 @@ __global__ void k_step_reduce(Layout L, Params P, const T* U, const T* A, const 
 
 // SolidInactive nodes keep their initial primitive values (never written by the reference); their
 // density is whatever the result buffer holds there.
+// Block blk of nblk over the list; writes one partial at out.
 template <class T>
-__global__ void k_inactive_reduce(Layout L, Params P, const T* A, const int64_t* nodes, const double* icPrim, int64_t n,
-                                  double* partial) {
-    __shared__ double sh[8];
+__device__ __forceinline__ void inactiveReduceBlock(const Layout& L, const Params& P, const T* A, const int64_t* nodes,
+                                                    const double* icPrim, int64_t n, int blk, int nblk, double* out,
+                                                    double* sh) {
     const double vm = fmax(4. / 3, P.gamma / P.Pr);
     double mx[4] = {0, 0, 0, 0};
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t q = (int64_t)blk * blockDim.x + threadIdx.x; q < n; q += (int64_t)nblk * blockDim.x) {
         int64_t m = nodes[q];
         int i = (int)(m % L.NI);
         int64_t r = m / L.NI;
@@ -121,7 +123,6 @@ __global__ void k_inactive_reduce(Layout L, Params P, const T* A, const int64_t*
         mx[2] = fmax(mx[2], cs + fabs(ic[2]));
         mx[3] = fmax(mx[3], ic[4] / (rho + 1) * vm);
     }
-    double* out = partial + (size_t)blockIdx.x * NP;
     for (int q = 0; q < 4; ++q) {
         double v = blockMax(mx[q], sh);
         if (threadIdx.x == 0) out[q] = v;
@@ -130,11 +131,18 @@ __global__ void k_inactive_reduce(Layout L, Params P, const T* A, const int64_t*
         for (int q = 4; q < NP; ++q) out[q] = 0;
 }
 
+template <class T>
+__global__ void k_inactive_reduce(Layout L, Params P, const T* A, const int64_t* nodes, const double* icPrim, int64_t n,
+                                  double* partial) {
+    __shared__ double sh[8];
+    inactiveReduceBlock(L, P, A, nodes, icPrim, n, blockIdx.x, gridDim.x, partial + (size_t)blockIdx.x * NP, sh);
+}
+
 // Combines the partials and advances t, timeLevel, dt. mode 0: initial dt only (Solver::initialize);
 // mode 1: end of a step.
-__global__ void k_finalize(Params P, const double* partials, int nPartials, int mode, int convHistory, int stopEndTime,
-                           double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap) {
-    __shared__ double sh[8];
+__device__ __forceinline__ void finalizeCase(const Params& P, const double* partials, int nPartials, int mode,
+                                             int convHistory, int stopEndTime, double tEnd, double cfl, double sigma,
+                                             StepScalars* st, double* hist, int histCap, double* sh) {
     double v[NP];
     for (int q = 0; q < NP; ++q) v[q] = 0;
     for (int b = threadIdx.x; b < nPartials; b += blockDim.x) {
@@ -168,6 +176,12 @@ __global__ void k_finalize(Params P, const double* partials, int nPartials, int 
     st->dt = dt;
 }
 
+__global__ void k_finalize(Params P, const double* partials, int nPartials, int mode, int convHistory, int stopEndTime,
+                           double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap) {
+    __shared__ double sh[8];
+    finalizeCase(P, partials, nPartials, mode, convHistory, stopEndTime, tEnd, cfl, sigma, st, hist, histCap, sh);
+}
+
 struct FaceSet {
     EdgeFace f[6];
     int64_t start[7];  // prefix sums of the owned-node counts
@@ -175,17 +189,18 @@ struct FaceSet {
 
 // Owned face nodes, ghosts and filter nodes after the final boundary application of a step.
 template <class T>
-__global__ void k_boundary_reduce(Layout L, Params P, Window W, FaceSet fs, const int64_t* ghostNode, int64_t nGhost,
-                                  const int64_t* filterList, int64_t nFilter, const T* U, const T* A, const uint8_t* cls,
-                                  const T* rhoPre, const T* rhoEPre, int norms, double* partials) {
-    __shared__ double sh[8];
+__device__ __forceinline__ void boundaryReduceBlock(const Layout& L, const Params& P, const Window& W, const FaceSet& fs,
+                                                    const int64_t* ghostNode, int64_t nGhost, const int64_t* filterList,
+                                                    int64_t nFilter, const T* U, const T* A, const uint8_t* cls,
+                                                    const T* rhoPre, const T* rhoEPre, int norms, int blk, int nblk,
+                                                    double* out, double* sh) {
     const size_t F = L.fstride;
     Eos<T> e = Eos<T>::make(P);
     const double vm = fmax(4. / 3, P.gamma / P.Pr);
     const int64_t nFace = fs.start[6];
     const int64_t total = nFace + nGhost + nFilter;
     double mx[4] = {0, 0, 0, 0}, sm[5] = {0, 0, 0, 0, 0}, bad = 0;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t q = (int64_t)blk * blockDim.x + threadIdx.x; q < total; q += (int64_t)nblk * blockDim.x) {
         size_t o;
         bool filt = false;
         size_t w = 0;
@@ -236,7 +251,6 @@ __global__ void k_boundary_reduce(Layout L, Params P, Window W, FaceSet fs, cons
         mx[2] = fmax(mx[2], cs + fabs((double)ww));
         mx[3] = fmax(mx[3], (double)mu / (a[0] + 1) * vm);
     }
-    double* out = partials + (size_t)blockIdx.x * NP;
     for (int q = 0; q < 4; ++q) {
         double v = blockMax(mx[q], sh);
         if (threadIdx.x == 0) out[q] = v;
@@ -249,7 +263,81 @@ __global__ void k_boundary_reduce(Layout L, Params P, Window W, FaceSet fs, cons
     if (threadIdx.x == 0) out[9] = b;
 }
 
+template <class T>
+__global__ void k_boundary_reduce(Layout L, Params P, Window W, FaceSet fs, const int64_t* ghostNode, int64_t nGhost,
+                                  const int64_t* filterList, int64_t nFilter, const T* U, const T* A, const uint8_t* cls,
+                                  const T* rhoPre, const T* rhoEPre, int norms, double* partials) {
+    __shared__ double sh[8];
+    boundaryReduceBlock(L, P, W, fs, ghostNode, nGhost, filterList, nFilter, U, A, cls, rhoPre, rhoEPre, norms,
+                        blockIdx.x, gridDim.x, partials + (size_t)blockIdx.x * NP, sh);
+}
+
+// ---- batched forms: block b belongs to case findCase(blockStart, b); partial slots of a case follow its
+// stage-4 block partials (boundary blocks first, then untouched-solid blocks)
+template <class T>
+__global__ void k_boundary_reduce_batched(const CaseView<T>* __restrict__ cv, const int* __restrict__ blockStart,
+                                          int ncases, int parity) {
+    __shared__ double sh[8];
+    const int b = blockIdx.x;
+    const int c = findCase(blockStart, ncases, b);
+    const CaseView<T>& v = cv[c];
+    FaceSet fs;
+    for (int f = 0; f < 6; ++f) fs.f[f] = v.faces[f];
+    for (int f = 0; f < 7; ++f) fs.start[f] = v.faceStart[f];
+    const int local = b - blockStart[c];
+    boundaryReduceBlock(v.L, v.P, v.W, fs, v.G.node, v.nGhost, v.filterList, v.nFilter,
+                        (const T*)roleBuffer(v, 0, parity), (const T*)roleBuffer(v, 1, parity), v.cls,
+                        (const T*)v.rhoPre, (const T*)v.rhoEPre, v.norms, local, v.nBoundaryBlocks,
+                        v.partials + (size_t)(v.nStage4 + local) * NP, sh);
+}
+
+template <class T>
+__global__ void k_inactive_reduce_batched(const CaseView<T>* __restrict__ cv, const int* __restrict__ blockStart,
+                                          int ncases, int parity) {
+    __shared__ double sh[8];
+    const int b = blockIdx.x;
+    const int c = findCase(blockStart, ncases, b);
+    const CaseView<T>& v = cv[c];
+    const int local = b - blockStart[c];
+    inactiveReduceBlock(v.L, v.P, (const T*)roleBuffer(v, 1, parity), v.inactiveList, v.inactiveIC, v.nInactive, local,
+                        v.nInactiveBlocks, v.partials + (size_t)(v.nStage4 + v.nBoundaryBlocks + local) * NP, sh);
+}
+
+template <class T>
+__global__ void k_finalize_batched(const CaseView<T>* __restrict__ cv) {
+    __shared__ double sh[8];
+    const CaseView<T>& v = cv[blockIdx.x];
+    finalizeCase(v.P, v.partials, v.nStage4 + v.nBoundaryBlocks + v.nInactiveBlocks, 1, v.convHistory, v.stopEndTime,
+                 v.tEnd, v.cfl, v.sigma, v.scal, v.hist, v.histCap, sh);
+}
+
 }  // namespace
+
+template <class T>
+void launchBoundaryReduceBatched(hipStream_t s, const CaseView<T>* cv, const int* blockStart, int totalBlocks, int ncases,
+                                 int parity) {
+    if (totalBlocks > 0)
+        hipLaunchKernelGGL(k_boundary_reduce_batched<T>, dim3(totalBlocks), dim3(RB), 0, s, cv, blockStart, ncases, parity);
+}
+
+template <class T>
+void launchInactiveReduceBatched(hipStream_t s, const CaseView<T>* cv, const int* blockStart, int totalBlocks, int ncases,
+                                 int parity) {
+    if (totalBlocks > 0)
+        hipLaunchKernelGGL(k_inactive_reduce_batched<T>, dim3(totalBlocks), dim3(RB), 0, s, cv, blockStart, ncases, parity);
+}
+
+template <class T>
+void launchFinalizeBatched(hipStream_t s, const CaseView<T>* cv, int ncases) {
+    hipLaunchKernelGGL(k_finalize_batched<T>, dim3(ncases), dim3(RB), 0, s, cv);
+}
+
+template void launchBoundaryReduceBatched<float>(hipStream_t, const CaseView<float>*, const int*, int, int, int);
+template void launchBoundaryReduceBatched<double>(hipStream_t, const CaseView<double>*, const int*, int, int, int);
+template void launchInactiveReduceBatched<float>(hipStream_t, const CaseView<float>*, const int*, int, int, int);
+template void launchInactiveReduceBatched<double>(hipStream_t, const CaseView<double>*, const int*, int, int, int);
+template void launchFinalizeBatched<float>(hipStream_t, const CaseView<float>*, int);
+template void launchFinalizeBatched<double>(hipStream_t, const CaseView<double>*, int);
 
 template <class T>
 int launchBoundaryReduce(hipStream_t s, const Layout& L, const Params& P, const Window& W, const EdgeFace faces[6],

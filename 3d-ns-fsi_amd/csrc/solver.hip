@@ -317,6 +317,7 @@ void Case::allocFields() {
     ftmp = alloc(2 * std::max<int64_t>(1, nFilter) * sizeof(T));
     nPartials = 1024;
     size_t slots = (size_t)std::max(nPartials, fusedMaxBlocks(L)) + 512 + 64;
+    partialSlots = slots;
     partials = (double*)alloc(slots * reducePartialSlots() * sizeof(double));
     scal = (StepScalars*)alloc(sizeof(StepScalars));
     hist = (double*)alloc((size_t)histCap * 5 * sizeof(double));
@@ -519,7 +520,8 @@ void Case::setOption(const std::string& name, int64_t value) {
         if (!ok) throw std::invalid_argument("tile_ty must be one of the compiled tile heights");
         tile.ty = (int)value;
     } else if (name == "tile_kc") {
-        if (value < 2 || value > 4096) throw std::invalid_argument("tile_kc out of range");
+        // the partial-slot buffer of the in-kernel scans is sized for z-chunks of >= kMinTileKc planes
+        if (value < kMinTileKc || value > 4096) throw std::invalid_argument("tile_kc out of range (8..4096)");
         tile.kc = (int)value;
     } else if (name == "autotune") {
         if (!initialized) throw StateError("autotune before init");

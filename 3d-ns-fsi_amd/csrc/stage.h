@@ -56,6 +56,7 @@ struct FusedTile {
     int ty = 8;
     int kc = 64;
 };
+constexpr int kMinTileKc = 8;  // smallest z-chunk accepted (sizes the scan partial buffers)
 // Stage 4 with partials != null also writes one partial per block of the end-of-step scans over
 // interior, non-filter nodes (norms of change when norms != 0, dt maxima, non-finite count). Returns the
 // number of blocks (= partial slots written).
@@ -70,6 +71,15 @@ int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P,
                      const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
                      const int32_t* filterFlag, uint16_t* comp);
 const int* fusedTileHeights(int* n);
+// Batched FP32 fused stage: fusedArgsBytes() bytes of arguments per case (buildFusedArgs fills them and
+// returns the case's tile count); launchStageBatched runs stage `stage` of every case in one launch over
+// a device table of those blocks, start = device prefix sums of the tile counts (ncases + 1 entries).
+size_t fusedArgsBytes();
+int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const Window& W, const float* Sin,
+                   const float* U, float* Sout, float* A, const uint8_t* cls, const float* rhoPre, const float* rhoEPre,
+                   const double* dtdev, FusedTile tile, double* partials, int norms, uint16_t* comp);
+void launchStageBatched(hipStream_t s, int stage, int ty, bool scan, bool kahan, const void* tab, const int* start,
+                        int ncases, int total);
 // Force on the body from the state S after a step's boundary application (FP64; the discrete momentum
 // balance over the faces between fluid interior and ghost nodes): appends (timeLevel, Fx, Fy, Fz) to
 // `ring` (cap entries of 4) and advances st->forceCount. partials: bodyForceBlocks(nGhost) x 3 doubles.
@@ -78,7 +88,7 @@ template <class T>
 void launchBodyForce(hipStream_t s, const Layout& L, const Params& P, const Window& W, const T* S, const uint8_t* cls,
                      const T* rhoPre, const T* rhoEPre, const int64_t* ghosts, int64_t nGhost, double* partials,
                      StepScalars* st, double* ring, int cap);
-int fusedMaxBlocks(const Layout& L);  // largest block count over all tile shapes
+int fusedMaxBlocks(const Layout& L);  // largest block count over all accepted tile shapes
 
 // End-of-step scans over what the stage-4 pass leaves out, after the boundary conditions: the owned
 // nodes of the six faces, the ghost nodes and the filter nodes. Returns the partial slots written.

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <stdexcept>
 
 #include "device_common.h"
@@ -280,16 +281,11 @@ __device__ __forceinline__ void readFaceNb(const Smem<T, TY>& sm, int r, int lan
     nb.Tt = sm.face[9][r][lane];
 }
 
-// FP32: at least 4 waves per SIMD (<= 128 VGPRs); FP64: at least 2.
+// One tile (block lb of the case described by a): the whole stage for 62 x TY output columns over one
+// z-chunk.
 template <class T, int STAGE, int TY, bool SCAN, bool KAHAN>
-__global__ __launch_bounds__(64 * (TY + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
-void k_stage_fused(FusedArgs<T> a) {
-    __shared__ Smem<T, TY> sm;
-    // XCD-aware mapping: hardware block p runs on XCD p % 8; give each XCD a contiguous range of tiles
-    const int p = blockIdx.x;
-    const int per = (a.nblocks + 7) / 8;
-    const int lb = (p % 8) * per + p / 8;
-    if (lb >= a.nblocks) return;
+__device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, Smem<T, TY>& sm,
+                                          double (&wred)[TY + 2][10]) {
     const int bx = lb % a.nbx;
     const int by = (lb / a.nbx) % a.nby;
     const int bz = lb / (a.nbx * a.nby);
@@ -489,7 +485,6 @@ void k_stage_fused(FusedArgs<T> a) {
     }
     if (SCAN) {
         // block partials: wave reduction, then one value per wave through LDS
-        __shared__ double wred[TY + 2][10];
         for (int q = 0; q < 10; ++q) {
             double v = (double)red[q];
             if (q < 4) v = wave_max(v);
@@ -503,6 +498,41 @@ void k_stage_fused(FusedArgs<T> a) {
             a.part[(size_t)lb * 10 + lane] = v;
         }
     }
+}
+
+// FP32: at least 4 waves per SIMD (<= 128 VGPRs); FP64: at least 2.
+template <class T, int STAGE, int TY, bool SCAN, bool KAHAN>
+__global__ __launch_bounds__(64 * (TY + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+void k_stage_fused(FusedArgs<T> a) {
+    __shared__ Smem<T, TY> sm;
+    __shared__ double wred[TY + 2][10];
+    // XCD-aware mapping: hardware block p runs on XCD p % 8; give each XCD a contiguous range of tiles
+    const int p = blockIdx.x;
+    const int per = (a.nblocks + 7) / 8;
+    const int lb = (p % 8) * per + p / 8;
+    if (lb >= a.nblocks) return;
+    stageTile<T, STAGE, TY, SCAN, KAHAN>(a, lb, sm, wred);
+}
+
+// Many independent cases in one launch: the tiles of case c are the global tiles start[c] ..
+// start[c+1]-1 (XCD-aware order as above, so each XCD sweeps a contiguous run of cases and tiles);
+// tab[c] holds the case's arguments for this stage (its own layout, coefficients, buffers, dt).
+template <class T, int STAGE, int TY, bool SCAN, bool KAHAN>
+__global__ __launch_bounds__(64 * (TY + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+void k_stage_batched(const FusedArgs<T>* __restrict__ tab, const int* __restrict__ start, int ncases, int total) {
+    __shared__ Smem<T, TY> sm;
+    __shared__ double wred[TY + 2][10];
+    const int p = blockIdx.x;
+    const int per = (total + 7) / 8;
+    const int g = (p % 8) * per + p / 8;
+    if (g >= total) return;
+    int lo = 0, hi = ncases - 1;  // last case whose first tile is <= g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    stageTile<T, STAGE, TY, SCAN, KAHAN>(tab[lo], g - start[lo], sm, wred);
 }
 
 // ---- force on the body: the discrete momentum balance of the flux form. Summed over all fluid interior
@@ -644,13 +674,7 @@ int fusedBlocks(const Layout& L, int kc) {
 }
 
 template <class T, int TY, bool K>
-void launchTYK(hipStream_t s, int stage, const FusedArgs<T>& a0, int kc) {
-    FusedArgs<T> a = a0;
-    a.nbx = (a.L.NI + 61) / 62;
-    a.nby = (a.L.NJ + TY - 1) / TY;
-    a.kc = kc;
-    a.nbz = (a.L.NK + a.kc - 1) / a.kc;
-    a.nblocks = a.nbx * a.nby * a.nbz;
+void launchTYK(hipStream_t s, int stage, const FusedArgs<T>& a) {
     const int grid = (a.nblocks + 7) / 8 * 8;
     dim3 blk(64, TY + 2);
     switch (stage) {
@@ -665,24 +689,40 @@ void launchTYK(hipStream_t s, int stage, const FusedArgs<T>& a0, int kc) {
 }
 
 template <class T, int TY>
-void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a, int kc) {
+void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a) {
     if constexpr (sizeof(T) == 4) {
         if (a.comp[0]) {
-            launchTYK<T, TY, true>(s, stage, a, kc);
+            launchTYK<T, TY, true>(s, stage, a);
             return;
         }
     }
-    launchTYK<T, TY, false>(s, stage, a, kc);
+    launchTYK<T, TY, false>(s, stage, a);
 }
 
-}  // namespace
+template <class T, int TY, bool K>
+void launchBatchedTYK(hipStream_t s, int stage, bool scan, const FusedArgs<T>* tab, const int* start, int ncases,
+                      int total) {
+    const int grid = (total + 7) / 8 * 8;
+    dim3 blk(64, TY + 2);
+    switch (stage) {
+        case 1: hipLaunchKernelGGL((k_stage_batched<T, 1, TY, false, K>), dim3(grid), blk, 0, s, tab, start, ncases, total); break;
+        case 2: hipLaunchKernelGGL((k_stage_batched<T, 2, TY, false, K>), dim3(grid), blk, 0, s, tab, start, ncases, total); break;
+        case 3: hipLaunchKernelGGL((k_stage_batched<T, 3, TY, false, K>), dim3(grid), blk, 0, s, tab, start, ncases, total); break;
+        default:
+            if (scan) hipLaunchKernelGGL((k_stage_batched<T, 4, TY, true, K>), dim3(grid), blk, 0, s, tab, start, ncases, total);
+            else hipLaunchKernelGGL((k_stage_batched<T, 4, TY, false, K>), dim3(grid), blk, 0, s, tab, start, ncases, total);
+            break;
+    }
+}
 
+int normTY(int ty) { return ty == 4 || ty == 6 || ty == 14 ? ty : 8; }
+
+// Every argument of one stage launch on one case, tile geometry included.
 template <class T>
-int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
-                     const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
-                     const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
-                     const int32_t* filterFlag, uint16_t* comp) {
-    FusedArgs<T> a{};
+void fillArgs(FusedArgs<T>& a, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin, const T* U,
+              T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre, const double* dtdev, FusedTile tile,
+              double* partials, int norms, const T* filteredRho, const int32_t* filterFlag, uint16_t* comp) {
+    a = FusedArgs<T>{};
     if (comp && sizeof(T) != 4) throw std::invalid_argument("compensated combine needs FP32 fields");
     a.fRho = stage == 1 ? filteredRho : nullptr;
     a.fFlag = stage == 1 ? filterFlag : nullptr;
@@ -704,16 +744,61 @@ int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P,
     a.rhoPre = rhoPre;
     a.rhoEPre = rhoEPre;
     a.dtdev = dtdev;
-    const int kc = tile.kc > 0 ? tile.kc : 64;
-    switch (tile.ty) {
-        case 4: launchTY<T, 4>(s, stage, a, kc); return fusedBlocks<T, 4>(L, kc);
-        case 6: launchTY<T, 6>(s, stage, a, kc); return fusedBlocks<T, 6>(L, kc);
-        case 14: launchTY<T, 14>(s, stage, a, kc); return fusedBlocks<T, 14>(L, kc);
-        default: launchTY<T, 8>(s, stage, a, kc); return fusedBlocks<T, 8>(L, kc);
+    const int ty = normTY(tile.ty);
+    a.kc = tile.kc > 0 ? tile.kc : 64;
+    a.nbx = (L.NI + 61) / 62;
+    a.nby = (L.NJ + ty - 1) / ty;
+    a.nbz = (L.NK + a.kc - 1) / a.kc;
+    a.nblocks = a.nbx * a.nby * a.nbz;
+}
+
+}  // namespace
+
+template <class T>
+int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
+                     const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
+                     const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
+                     const int32_t* filterFlag, uint16_t* comp) {
+    FusedArgs<T> a;
+    fillArgs(a, stage, L, P, W, Sin, U, Sout, A, cls, rhoPre, rhoEPre, dtdev, tile, partials, norms, filteredRho,
+             filterFlag, comp);
+    switch (normTY(tile.ty)) {
+        case 4: launchTY<T, 4>(s, stage, a); break;
+        case 6: launchTY<T, 6>(s, stage, a); break;
+        case 14: launchTY<T, 14>(s, stage, a); break;
+        default: launchTY<T, 8>(s, stage, a); break;
+    }
+    return a.nblocks;
+}
+
+size_t fusedArgsBytes() { return sizeof(FusedArgs<float>); }
+
+int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const Window& W, const float* Sin,
+                   const float* U, float* Sout, float* A, const uint8_t* cls, const float* rhoPre, const float* rhoEPre,
+                   const double* dtdev, FusedTile tile, double* partials, int norms, uint16_t* comp) {
+    FusedArgs<float> a;
+    fillArgs<float>(a, stage, L, P, W, Sin, U, Sout, A, cls, rhoPre, rhoEPre, dtdev, tile, partials, norms, nullptr,
+                    nullptr, comp);
+    std::memcpy(dst, &a, sizeof a);
+    return a.nblocks;
+}
+
+void launchStageBatched(hipStream_t s, int stage, int ty, bool scan, bool kahan, const void* tab, const int* start,
+                        int ncases, int total) {
+    const FusedArgs<float>* t = static_cast<const FusedArgs<float>*>(tab);
+    switch (normTY(ty)) {
+        case 4: kahan ? launchBatchedTYK<float, 4, true>(s, stage, scan, t, start, ncases, total)
+                      : launchBatchedTYK<float, 4, false>(s, stage, scan, t, start, ncases, total); break;
+        case 6: kahan ? launchBatchedTYK<float, 6, true>(s, stage, scan, t, start, ncases, total)
+                      : launchBatchedTYK<float, 6, false>(s, stage, scan, t, start, ncases, total); break;
+        case 14: kahan ? launchBatchedTYK<float, 14, true>(s, stage, scan, t, start, ncases, total)
+                       : launchBatchedTYK<float, 14, false>(s, stage, scan, t, start, ncases, total); break;
+        default: kahan ? launchBatchedTYK<float, 8, true>(s, stage, scan, t, start, ncases, total)
+                       : launchBatchedTYK<float, 8, false>(s, stage, scan, t, start, ncases, total); break;
     }
 }
 
-int fusedMaxBlocks(const Layout& L) { return fusedBlocks<float, 4>(L, 16); }
+int fusedMaxBlocks(const Layout& L) { return fusedBlocks<float, 4>(L, kMinTileKc); }
 
 int bodyForceBlocks(int64_t nGhost) { return (int)std::min<int64_t>(256, std::max<int64_t>(1, (nGhost + 255) / 256)); }
 

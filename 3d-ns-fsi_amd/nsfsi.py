@@ -114,6 +114,23 @@ def lib():
     L.ns_destroy.argtypes = [H]
     L.ns_destroy.restype = None
     L.ns_abi_version.restype = C.c_int32
+    B = C.c_void_p
+    L.ns_batch_create.argtypes = [C.POINTER(NSConfig), C.c_int32, C.POINTER(B)]
+    L.ns_batch_init.argtypes = [B]
+    L.ns_batch_step.argtypes = [B, C.c_int32]
+    L.ns_batch_get_time.argtypes = [B, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+    L.ns_batch_get_field.argtypes = [B, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    L.ns_batch_norm_history.argtypes = [B, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.c_int64,
+                                        C.POINTER(C.c_int64)]
+    L.ns_batch_set_option.argtypes = [B, C.c_char_p, C.c_int64]
+    L.ns_batch_get_tile.argtypes = [B, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.ns_batch_last_error.argtypes = [B]
+    L.ns_batch_last_error.restype = C.c_char_p
+    L.ns_batch_destroy.argtypes = [B]
+    L.ns_batch_destroy.restype = None
+    for name in ("ns_batch_create", "ns_batch_init", "ns_batch_step", "ns_batch_get_time", "ns_batch_get_field",
+                 "ns_batch_norm_history", "ns_batch_set_option", "ns_batch_get_tile"):
+        getattr(L, name).restype = C.c_int
     for name in ("ns_config_parse", "ns_create", "ns_init", "ns_set_initial_field", "ns_step", "ns_run", "ns_get_time",
                  "ns_get_field", "ns_get_node_types", "ns_get_counts", "ns_get_ghosts", "ns_get_filter_nodes",
                  "ns_norm_history", "ns_force_history", "ns_totals", "ns_get_geometry", "ns_profile", "ns_set_option", "ns_get_tile"):
@@ -380,6 +397,84 @@ class Solver:
     def close(self):
         if self._h:
             lib().ns_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent FP32 cases advanced together (ns_batch_*): each kernel of the step runs once for
+    all cases, each case at its own dt. ``configs`` is a sequence of NSConfig."""
+
+    def __init__(self, configs):
+        self.configs = list(configs)
+        arr = (NSConfig * len(self.configs))(*self.configs)
+        self._h = C.c_void_p()
+        s = lib().ns_batch_create(arr, len(self.configs), C.byref(self._h))
+        if s != 0:
+            msg = lib().ns_batch_last_error(self._h).decode() if self._h else ""
+            lib().ns_batch_destroy(self._h)
+            self._h = C.c_void_p()
+            raise NSError(s, msg)
+
+    def _check(self, s):
+        if s != 0:
+            raise NSError(s, lib().ns_batch_last_error(self._h).decode())
+
+    def __len__(self):
+        return len(self.configs)
+
+    def initialize(self):
+        self._check(lib().ns_batch_init(self._h))
+
+    def step(self, nsteps: int = 1):
+        self._check(lib().ns_batch_step(self._h, int(nsteps)))
+
+    def time(self, c: int):
+        t, lv, dt = C.c_double(), C.c_uint64(), C.c_double()
+        self._check(lib().ns_batch_get_time(self._h, int(c), C.byref(t), C.byref(lv), C.byref(dt)))
+        return t.value, lv.value, dt.value
+
+    def field(self, c: int, name) -> np.ndarray:
+        k = self.configs[c]
+        which = FIELDS.index(name) if isinstance(name, str) else int(name)
+        out = np.empty(k.NI * k.NJ * k.NK)
+        self._check(lib().ns_batch_get_field(self._h, int(c), which, _dptr(out)))
+        return out.reshape(k.NK, k.NJ, k.NI)
+
+    def conserved(self, c: int) -> np.ndarray:
+        return np.stack([self.field(c, n) for n in FIELDS[:5]])
+
+    def getConvergenceHistory(self, c: int):
+        out = {}
+        for q, name in enumerate(FIELDS[:5]):
+            cnt = C.c_int64()
+            self._check(lib().ns_batch_norm_history(self._h, int(c), q, None, 0, C.byref(cnt)))
+            a = np.zeros(cnt.value)
+            if cnt.value:
+                self._check(lib().ns_batch_norm_history(self._h, int(c), q, _dptr(a), cnt.value, C.byref(cnt)))
+            out[name] = a
+        return out
+
+    def setOption(self, name: str, value: int):
+        self._check(lib().ns_batch_set_option(self._h, name.encode(), int(value)))
+
+    def tile(self):
+        ty, kc = C.c_int32(), C.c_int32()
+        self._check(lib().ns_batch_get_tile(self._h, C.byref(ty), C.byref(kc)))
+        return ty.value, kc.value
+
+    def autotune(self, reps: int = 3):
+        self.setOption("autotune", reps)
+        return self.tile()
+
+    def close(self):
+        if self._h:
+            lib().ns_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

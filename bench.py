@@ -61,6 +61,76 @@ def copy_calibration(nbytes):
     hip.hipFree(dst)
 
 
+def bench_sweep(a):
+    """The batched sweep: 64 independent FP32 cases (grid edges 32..128, own body radius, Mach, Re and dt)
+    advanced together by ns_batch_step, against the same cases stepped one after another (each its own
+    solver replaying its own step graph, same tile shape)."""
+    import cases
+    import nsfsi
+
+    cfgs = cases.sweep64(steps=a.steps + a.warmup)
+    for c in cfgs:
+        c.conv_history = 2
+        c.kahan = 0 if a.no_kahan else 1
+    nodes = sum(c.NI * c.NJ * c.NK for c in cfgs)
+    b = nsfsi.Batch(cfgs)
+    b.initialize()
+    if a.tile:
+        ty, kc = (int(x) for x in a.tile.split(","))
+        b.setOption("tile_ty", ty)
+        b.setOption("tile_kc", kc)
+        tile = b.tile()
+    else:
+        tile = b.autotune(2)
+    b.setOption("graph", 0 if a.no_graph else 1)
+    b.step(a.warmup)
+    t0 = time.perf_counter()
+    b.step(a.steps)  # synchronises
+    el = time.perf_counter() - t0
+    b.close()
+    value = nodes * a.steps / el / 1e6
+    # the same cases one at a time
+    seq = 0.0
+    for cfg in cfgs:
+        s = nsfsi.Solver(cfg)
+        s.initialize()
+        s.setOption("tile_ty", tile[0])
+        s.setOption("tile_kc", tile[1])
+        s.setOption("graph", 0 if a.no_graph else 1)
+        s.marchTimeStep(2)
+        t1 = time.perf_counter()
+        s.marchTimeStep(a.steps)
+        seq += time.perf_counter() - t1
+        s.close()
+    seq_value = nodes * a.steps / seq / 1e6
+    ms_step = el / a.steps * 1e3
+    line = {
+        "metric": "M node-updates/s per RK4 step",
+        "value": round(value, 3),
+        "unit": "M node-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (64 seeded cases: freestream + Gaussian perturbation)",
+        "config": {"workload": "sweep64", "cases": len(cfgs), "nodes": nodes,
+                   "grid_edges": sorted({c.NI for c in cfgs}), "compensated_combine": not a.no_kahan,
+                   "tile_rows_x_zplanes": list(tile), "hip_graph": not a.no_graph},
+        "roofline": {"bound": "hbm", "achieved": round(nodes * VALUES_PER_NODE_STEP * 4 / (ms_step * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(nodes * VALUES_PER_NODE_STEP * 4 / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "whole batched step (algorithmic bytes / step time)"},
+        "sequential_per_case": {"value": round(seq_value, 3), "ms_per_step": round(seq / a.steps * 1e3, 4),
+                                "speedup_of_batch": round(value / seq_value, 3)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +153,10 @@ def main():
         print("bench.py measures one GPU: this build's path runs on a single MI355X (no multi-GPU sharding); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "sweep64":
+        bench_sweep(a)
+        return
 
     import cases
     import nsfsi

@@ -209,6 +209,37 @@ const char* ns_last_error(ns_handle* h);
 /* Releases all device memory. Null is accepted. */
 void ns_destroy(ns_handle* h);
 
+/* ---- Batched cases (no reference counterpart: the reference runs one case per process).
+ * Many independent cases (each its own ns_config: grid size, body, Mach, Re, initial field, stability
+ * limits) advance together on one GPU; every kernel of the time step runs once for all cases, each case
+ * keeps its own dt / t / timeLevel (src/Solver.cpp:80-126 per case), and a step replays as one captured
+ * HIP graph. Results of each case equal those of ns_step on that case alone with the same tile shape.
+ * Restrictions: precision fp32, the same kahan setting in every case, no periodic density filter
+ * (NS_ERR_UNSUPPORTED otherwise). */
+typedef struct ns_batch ns_batch;
+
+/* Creates the cases (like ns_create for each config). On error *out still receives a handle that holds
+ * the error text; destroy it with ns_batch_destroy. */
+ns_status ns_batch_create(const ns_config* cfgs, int32_t n, ns_batch** out);
+/* Solver::initialize of every case (classification, ghost geometry, initial field, first dt). */
+ns_status ns_batch_init(ns_batch* b);
+/* nsteps time steps of every case (each at its own dt). NS_ERR_NONFINITE if any case holds a
+ * non-finite value afterwards. */
+ns_status ns_batch_step(ns_batch* b, int32_t nsteps);
+/* Time, time level and next dt of case c. */
+ns_status ns_batch_get_time(ns_batch* b, int32_t c, double* t, uint64_t* level, double* dt);
+/* Field `which` of case c (ids and layout as ns_get_field). */
+ns_status ns_batch_get_field(ns_batch* b, int32_t c, int32_t which, double* out);
+/* Norm-of-change history of case c (as ns_norm_history). */
+ns_status ns_batch_norm_history(ns_batch* b, int32_t c, int32_t var, double* out, int64_t n, int64_t* count);
+/* Options: "graph" (1 default), "tile_ty" / "tile_kc" (tile shape of the batched stage, shared by all
+ * cases), "autotune" (time every tile shape on this batch with `value` launches each and keep the
+ * fastest; states unchanged). */
+ns_status ns_batch_set_option(ns_batch* b, const char* name, int64_t value);
+ns_status ns_batch_get_tile(ns_batch* b, int32_t* ty, int32_t* kc);
+const char* ns_batch_last_error(ns_batch* b);
+void ns_batch_destroy(ns_batch* b);
+
 /* Library ABI version, NS_FSI_ABI_VERSION. */
 int32_t ns_abi_version(void);
 

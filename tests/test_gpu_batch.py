@@ -1,0 +1,95 @@
+"""Batched cases (ns_batch_*) against the same cases run one at a time.
+
+The batched kernels run the per-node arithmetic of the single-case kernels (same device functions, same
+tile shape), each case with its own dt, so every case's state, time and dt after N steps must equal its
+single-case run bit for bit; the norm histories differ only in the order of their FP64 sums.
+"""
+import numpy as np
+import pytest
+
+import cases
+import nsfsi
+
+pytestmark = pytest.mark.gpu
+
+
+def small_sweep(n=6, steps=8):
+    cfgs = cases.sweep64(n=n, steps=steps)
+    # a second body shape and boundary set in the mix: z-cylinder case with periodic z
+    cyl = cases.cylinder256(N=40, steps=steps, precision="fp32")
+    cyl.NK = 20
+    cyl.bc[4] = cyl.bc[5] = nsfsi.BC["periodic"]
+    return cfgs + [cyl]
+
+
+def single(cfg, steps, tile):
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    s.setOption("tile_ty", tile[0])
+    s.setOption("tile_kc", tile[1])
+    s.marchTimeStep(steps)
+    out = (s.mesh.conserved(), s.time(), s.getConvergenceHistory())
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("graph", [1, 0])
+def test_batch_equals_single_cases(graph):
+    steps = 8
+    cfgs = small_sweep(steps=steps)
+    print("batch of", [(c.NI, c.NJ, c.NK) for c in cfgs], "graph", graph, flush=True)
+    b = nsfsi.Batch(cfgs)
+    b.initialize()
+    b.setOption("graph", graph)
+    tile = b.tile()
+    b.step(3)
+    b.step(steps - 3)  # two calls: the buffer parity carries over
+    for c, cfg in enumerate(cfgs):
+        U, (t, lv, dt), hist = single(cfg, steps, tile)
+        bt, blv, bdt = b.time(c)
+        assert blv == lv == steps
+        assert bt == t and bdt == dt, (c, bt, t, bdt, dt)
+        assert np.array_equal(b.conserved(c), U), c
+        bh = b.getConvergenceHistory(c)
+        for name in nsfsi.FIELDS[:5]:
+            assert np.allclose(bh[name], hist[name], rtol=1e-12, atol=0)
+    b.close()
+
+
+def test_batch_cases_advance_at_their_own_dt():
+    cfgs = small_sweep(n=4, steps=5)
+    b = nsfsi.Batch(cfgs)
+    b.initialize()
+    b.step(5)
+    dts = [b.time(c)[2] for c in range(len(cfgs))]
+    assert len(set(dts)) == len(dts)  # different grids, Mach numbers, Reynolds numbers
+    b.close()
+
+
+def test_batch_autotune_keeps_results():
+    cfgs = small_sweep(n=4, steps=4)
+    out = []
+    for tune in (0, 1):
+        b = nsfsi.Batch(cfgs)
+        b.initialize()
+        if tune:
+            b.autotune(1)
+        b.step(4)
+        out.append([b.conserved(c) for c in range(len(cfgs))])
+        b.close()
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a, c)
+
+
+def test_batch_plain_combine():
+    cfgs = small_sweep(n=3, steps=4)
+    for c in cfgs:
+        c.kahan = 0
+    b = nsfsi.Batch(cfgs)
+    b.initialize()
+    tile = b.tile()
+    b.step(4)
+    for c, cfg in enumerate(cfgs):
+        U, _, _ = single(cfg, 4, tile)
+        assert np.array_equal(b.conserved(c), U)
+    b.close()
