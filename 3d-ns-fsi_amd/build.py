@@ -26,6 +26,8 @@ SOURCES = [
     ("stage_fused.hip", False),
     ("reduce.hip", False),
     ("batch.cpp", False),
+    ("adjoint.hip", True),
+    ("adjoint_host.cpp", True),
 ]
 
 ARCH = "gfx950"

@@ -1,0 +1,55 @@
+/*
+ * adjoint.h — launchers of the discrete-adjoint kernels (adjoint.hip), FP64.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "ns_internal.h"
+
+namespace nsfsi {
+
+struct AdjConst {
+    double i2d[3];  // 1 / (2 h)
+    double c43[3];  // 2 / (3 h^2)
+    double c1[3];   // 1 / (2 h^2)
+    double gamma, gm1, H0, ScP1, Re, pf;
+};
+AdjConst makeAdjConst(const Params& P);
+
+// Transposed right-hand side at one stage state.
+struct AdjArgs {
+    Layout L;
+    Window W;
+    AdjConst c;
+    const double* S;       // stage state after its boundary application (5 fields)
+    const uint8_t* cls;
+    const double* Ppre;    // pre-filter density / energy of the filter nodes (window-dense)
+    const double* PEpre;
+    const double* lam;     // adjoint of the stage right-hand side (5 fields, zero at non-interior nodes)
+    double objCoef;        // d J / d Fx at this state (0: no objective term)
+    double vol;            // dx dy dz
+    double* passA;         // scratch, 19 fields
+    double* Sbar;          // += adjoint of S (5 fields)
+    double* Pbar;          // += adjoint of the pre-filter tables (window-dense)
+    double* PEbar;
+};
+int adjRhsPartials(const Layout& L);  // dJ/dRe partial slots written by launchAdjRhs
+void launchAdjRhs(hipStream_t s, const AdjArgs& a, double* rePartial);
+
+// Transposed boundary pieces. Sb is updated in place (it holds the adjoint of the state after the
+// piece on entry and of the state before it on return).
+void launchAdjGhost(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, double* Sb);
+void launchAdjFilter(hipStream_t s, const Layout& L, const Window& W, const int64_t* fn, int64_t n, double* Sb,
+                     double* tmp, const double* Pb, const double* PEb);
+void launchAdjEdge(hipStream_t s, const Layout& L, const Params& P, const EdgeFace& f, double* Sb, const double* S,
+                   double ts, double* qb);
+// Interior nodes: Ub += B, lam[r] += coef[r] * B (null entries skipped); other nodes: prev += B (if set).
+void launchAdjRoute(hipStream_t s, const Layout& L, const uint8_t* cls, const double* B, double* Ub, double* const lam[4],
+                    const double coef[4], double* prev);
+void launchAxpyWindow(hipStream_t s, double* y, const double* x, size_t n);
+void launchAdjTablesToState(hipStream_t s, const Layout& L, const Window& W, const int64_t* fn, int64_t n,
+                            const double* Pb, const double* PEb, double* Sb);
+
+}  // namespace nsfsi

@@ -206,6 +206,19 @@ ns_status ns_force_history(ns_handle* h, double* out, int64_t n, int64_t* count)
     });
 }
 
+ns_status ns_set_dt_schedule(ns_handle* h, const double* dts, int64_t n) {
+    return guarded(h, [&] { live(h).setDtSchedule(dts, n); });
+}
+
+ns_status ns_set_inflow_profile(ns_handle* h, const double* q, int64_t n) {
+    return guarded(h, [&] { live(h).setInflowProfile(q, n); });
+}
+
+ns_status ns_adjoint_drag(ns_handle* h, int32_t nsteps, int32_t n0, int32_t n1, int32_t checkpoint, double* J,
+                          double* grad_u0, double* grad_inflow, double* grad_re, double* dts) {
+    return guarded(h, [&] { live(h).adjointDrag(nsteps, n0, n1, checkpoint, J, grad_u0, grad_inflow, grad_re, dts); });
+}
+
 ns_status ns_totals(ns_handle* h, double out[5]) {
     return guarded(h, [&] {
         if (!out) throw std::invalid_argument("null output");

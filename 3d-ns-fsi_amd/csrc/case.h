@@ -99,6 +99,28 @@ public:
     bool bodyForce = false;
     double* forceRing = nullptr;
     std::vector<double> forceHost;
+    // prescribed step sizes (index = time level) instead of the stability limit; and the inlet profile
+    void setDtSchedule(const double* dts, int64_t n);
+    void setInflowProfile(const double* q, int64_t n);
+    const double* dtSchedule = nullptr;
+    int64_t dtScheduleLen = 0;
+    double* dtScheduleBuf = nullptr;
+    int64_t dtScheduleCap = 0;
+    double* inflowBuf = nullptr;
+    // Stage states of one step for the adjoint: S[0] = the step's input state and its pre-filter tables,
+    // S[1..3] = the outputs of stages 1-3 after their boundary application, with the tables of each.
+    struct StageTape {
+        double* S[4];
+        double* P[4];
+        double* PE[4];
+    };
+    // One FP64 step (kernel by kernel) recording its stage states into tape.
+    void stepTaped(StageTape* tape);
+    // Discrete adjoint of J = mean over time levels n0..n1 of C_D after nsteps steps from the current
+    // (initial) state, at the step sizes of the forward run (frozen); see ns_adjoint_drag.
+    void adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* gradU0, double* gradInflow,
+                     double* gradRe, double* dts);
+    double referenceArea() const;
     double t = 0, dt = 0;
     uint64_t level = 0;
     bool nonFinite = false;
@@ -114,6 +136,7 @@ private:
     template <class T> void applyBCs(T* S, double frac);
     template <class T> void stepOnce();
     std::vector<void*> allocations;
+    StageTape* tapeNow = nullptr;
     struct EventPair { hipEvent_t a, b; int group; };
     std::vector<EventPair> events;
     bool profiling = false;

@@ -335,7 +335,9 @@ __device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, c
         }
         acc += (T)nb * extrap(o);
         acc /= (T)(2 * nb);
-        T vel = (T)(fmin(1., ts / P.rampTime) * P.M_0);
+        double vn = fmin(1., ts / P.rampTime) * P.M_0;
+        if (P.inflow) vn *= P.inflow[(size_t)c[ax2] * (ax1 == 0 ? L.NI : L.NJ) + c[ax1]];
+        T vel = (T)vn;
         if (f.side == 1) vel *= (T)-1;
         // component selection by selects, not a three-way branch: gfx950 code generation has dropped the
         // last arm of such a branch on a kernel argument (the z-face case) in this function

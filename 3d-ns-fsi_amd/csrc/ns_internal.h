@@ -60,6 +60,9 @@ struct Params {
     double ox, oy, oz;         // origin offset in nodes
     double gamma, Pr, Re, ScPlusOne, M_0;
     double rampTime;
+    // optional inlet velocity profile: factor per node of the inlet plane (index a + n_a * b over the
+    // plane's two in-plane axes in (i, j, k) order), null = uniform (the reference's inlet)
+    const double* inflow;
 };
 
 // Owned node box of one edge boundary (src/Boundary.cpp:18-71), with its type and side.

@@ -140,9 +140,11 @@ __global__ void k_inactive_reduce(Layout L, Params P, const T* A, const int64_t*
 
 // Combines the partials and advances t, timeLevel, dt. mode 0: initial dt only (Solver::initialize);
 // mode 1: end of a step.
+// dtSched (optional): prescribed step sizes by time level, used instead of the stability limit.
 __device__ __forceinline__ void finalizeCase(const Params& P, const double* partials, int nPartials, int mode,
                                              int convHistory, int stopEndTime, double tEnd, double cfl, double sigma,
-                                             StepScalars* st, double* hist, int histCap, double* sh) {
+                                             StepScalars* st, double* hist, int histCap, double* sh,
+                                             const double* dtSched) {
     double v[NP];
     for (int q = 0; q < NP; ++q) v[q] = 0;
     for (int b = threadIdx.x; b < nPartials; b += blockDim.x) {
@@ -173,13 +175,16 @@ __device__ __forceinline__ void finalizeCase(const Params& P, const double* part
     double dtVisc = fabs(sigma) / (red[3] * (1 / dx2 + 1 / dy2 + 1 / dz2));
     double dt = fmin(dtConv, dtVisc);
     if (stopEndTime && t + dt > tEnd) dt = tEnd - t;
+    if (dtSched) dt = dtSched[st->level];
     st->dt = dt;
 }
 
 __global__ void k_finalize(Params P, const double* partials, int nPartials, int mode, int convHistory, int stopEndTime,
-                           double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap) {
+                           double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap,
+                           const double* dtSched) {
     __shared__ double sh[8];
-    finalizeCase(P, partials, nPartials, mode, convHistory, stopEndTime, tEnd, cfl, sigma, st, hist, histCap, sh);
+    finalizeCase(P, partials, nPartials, mode, convHistory, stopEndTime, tEnd, cfl, sigma, st, hist, histCap, sh,
+                 dtSched);
 }
 
 struct FaceSet {
@@ -308,7 +313,7 @@ __global__ void k_finalize_batched(const CaseView<T>* __restrict__ cv) {
     __shared__ double sh[8];
     const CaseView<T>& v = cv[blockIdx.x];
     finalizeCase(v.P, v.partials, v.nStage4 + v.nBoundaryBlocks + v.nInactiveBlocks, 1, v.convHistory, v.stopEndTime,
-                 v.tEnd, v.cfl, v.sigma, v.scal, v.hist, v.histCap, sh);
+                 v.tEnd, v.cfl, v.sigma, v.scal, v.hist, v.histCap, sh, nullptr);
 }
 
 }  // namespace
@@ -381,9 +386,10 @@ void launchInactiveReduce(hipStream_t s, const Layout& L, const Params& P, const
 }
 
 void launchFinalize(hipStream_t s, const Params& P, const double* partials, int nPartials, int mode, int convHistory,
-                    int stopEndTime, double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap) {
+                    int stopEndTime, double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap,
+                    const double* dtSched) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RB), 0, s, P, partials, nPartials, mode, convHistory, stopEndTime, tEnd,
-                       cfl, sigma, st, hist, histCap);
+                       cfl, sigma, st, hist, histCap, dtSched);
 }
 
 int reducePartialSlots() { return NP; }

@@ -410,7 +410,7 @@ void Case::computeDt(int mode) {
         npart += inactiveReduceBlocks(nInactive);
     }
     launchFinalize(stream, P, partials, npart, mode, cfg.conv_history, cfg.stop_criterion == 1, cfg.t_end,
-                   cfg.conv_stability_limit, cfg.visc_stability_limit, scal, hist, histCap);
+                   cfg.conv_stability_limit, cfg.visc_stability_limit, scal, hist, histCap, dtSchedule);
 }
 
 template <class T>
@@ -453,6 +453,14 @@ void Case::stepOnce() {
         launchFilterFlag(stream, fs, scal);
         launchRhoFilter<T>(stream, L, u, fRho, cls, fFlag);
     }
+    auto tapeCopy = [&](int slot, const T* S) {
+        if (!tapeNow) return;
+        const size_t fb = 5 * L.fstride * sizeof(T), wb = std::max<size_t>(1, W.size()) * sizeof(T);
+        hipCheck(hipMemcpyAsync(tapeNow->S[slot], S, fb, hipMemcpyDeviceToDevice, stream), "tape");
+        hipCheck(hipMemcpyAsync(tapeNow->P[slot], rhoPre, wb, hipMemcpyDeviceToDevice, stream), "tape");
+        hipCheck(hipMemcpyAsync(tapeNow->PE[slot], rhoEPre, wb, hipMemcpyDeviceToDevice, stream), "tape");
+    };
+    tapeCopy(0, u);
     for (int st = 1; st <= 4; ++st) {
         if (stageKernel == 1) {
             mark(0, true);
@@ -474,6 +482,7 @@ void Case::stepOnce() {
         mark(2, true);
         applyBCs<T>(out[st - 1], frac[st - 1]);
         mark(2, false);
+        if (st < 4) tapeCopy(st, out[st - 1]);
     }
     mark(3, true);
     if (stageKernel == 1 && fusedScans) {
@@ -490,7 +499,7 @@ void Case::stepOnce() {
             npart += inactiveReduceBlocks(nInactive);
         }
         launchFinalize(stream, P, partials, npart, 1, cfg.conv_history, cfg.stop_criterion == 1, cfg.t_end,
-                       cfg.conv_stability_limit, cfg.visc_stability_limit, scal, hist, histCap);
+                       cfg.conv_stability_limit, cfg.visc_stability_limit, scal, hist, histCap, dtSchedule);
     } else {
         computeDt<T>(1);
     }
@@ -651,6 +660,48 @@ void Case::setInitialField(const double* U5) {
     readScalars();
 }
 
+void Case::setDtSchedule(const double* dts, int64_t n) {
+    if (!initialized) throw StateError("dt schedule before init");
+    dropGraphs();
+    if (!dts || n <= 0) {
+        dtSchedule = nullptr;
+        dtScheduleLen = 0;
+        return;
+    }
+    if ((int64_t)level >= n) throw std::invalid_argument("dt schedule shorter than the current time level");
+    if (n > dtScheduleCap) {
+        dtScheduleBuf = (double*)alloc((size_t)n * sizeof(double));
+        dtScheduleCap = n;
+    }
+    hipCheck(hipMemcpy(dtScheduleBuf, dts, (size_t)n * sizeof(double), hipMemcpyHostToDevice), "copy dt schedule");
+    dtSchedule = dtScheduleBuf;
+    dtScheduleLen = n;
+    // the next step already uses the schedule
+    hipCheck(hipMemcpy(&scal->dt, dts + level, sizeof(double), hipMemcpyHostToDevice), "copy dt");
+    readScalars();
+}
+
+void Case::setInflowProfile(const double* q, int64_t n) {
+    if (!initialized) throw StateError("inflow profile before init");
+    dropGraphs();
+    int axis = -1;
+    for (const EdgeFace& f : faces)
+        if (f.type == NS_BC_INLET) {
+            if (axis >= 0 && axis != f.axis) throw std::invalid_argument("inlets on different axes");
+            axis = f.axis;
+        }
+    if (axis < 0) throw std::invalid_argument("no inlet boundary");
+    const int64_t need = axis == 0 ? (int64_t)L.NJ * L.NK : (axis == 1 ? (int64_t)L.NI * L.NK : (int64_t)L.NI * L.NJ);
+    if (!q) {
+        P.inflow = nullptr;
+        return;
+    }
+    if (n != need) throw std::invalid_argument("inflow profile size must equal the inlet plane's node count");
+    if (!inflowBuf) inflowBuf = (double*)alloc((size_t)need * sizeof(double));
+    hipCheck(hipMemcpy(inflowBuf, q, (size_t)need * sizeof(double), hipMemcpyHostToDevice), "copy inflow profile");
+    P.inflow = inflowBuf;
+}
+
 void Case::readScalars() {
     StepScalars s;
     hipCheck(hipMemcpyAsync(&s, scal, sizeof(s), hipMemcpyDeviceToHost, stream), "copy scalars");
@@ -690,6 +741,8 @@ void Case::dropGraphs() {
 
 void Case::step(int nsteps) {
     if (!initialized) throw StateError("step before init");
+    if (dtSchedule && (int64_t)level + nsteps >= dtScheduleLen)
+        throw std::invalid_argument("dt schedule too short for the requested steps");
     int done = 0;
     while (done < nsteps) {
         int batch = std::min(nsteps - done, histCap);
@@ -723,6 +776,16 @@ void Case::step(int nsteps) {
         readScalars();
         done += batch;
     }
+}
+
+void Case::stepTaped(StageTape* tape) {
+    if (!initialized) throw StateError("step before init");
+    if (fp32) throw Unsupported("the adjoint runs on the FP64 form");
+    tapeNow = tape;
+    stepOnce<double>();  // swaps U and A itself
+    tapeNow = nullptr;
+    hipCheck(hipGetLastError(), "taped step");
+    readScalars();
 }
 
 bool Case::filterDisabled() const {

@@ -105,8 +105,10 @@ void launchStepReduce(hipStream_t s, const Layout& L, const Params& P, const T* 
 template <class T>
 void launchInactiveReduce(hipStream_t s, const Layout& L, const Params& P, const T* A, const int64_t* nodes,
                           const double* icPrim, int64_t n, double* partial);
+// dtSched: optional device array of prescribed step sizes indexed by time level (replaces the limit).
 void launchFinalize(hipStream_t s, const Params& P, const double* partials, int nPartials, int mode, int convHistory,
-                    int stopEndTime, double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap);
+                    int stopEndTime, double tEnd, double cfl, double sigma, StepScalars* st, double* hist, int histCap,
+                    const double* dtSched = nullptr);
 int reducePartialSlots();
 int inactiveReduceBlocks(int64_t n);  // partial slots the inactive-node reduction writes (<= 64)
 

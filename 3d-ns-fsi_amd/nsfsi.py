@@ -105,6 +105,11 @@ def lib():
     L.ns_norm_history.argtypes = [H, C.c_int32, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]
     L.ns_totals.argtypes = [H, C.POINTER(C.c_double)]
     L.ns_force_history.argtypes = [H, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]
+    L.ns_set_dt_schedule.argtypes = [H, C.POINTER(C.c_double), C.c_int64]
+    L.ns_set_inflow_profile.argtypes = [H, C.POINTER(C.c_double), C.c_int64]
+    L.ns_adjoint_drag.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
     L.ns_get_geometry.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.ns_profile.argtypes = [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.ns_set_option.argtypes = [H, C.c_char_p, C.c_int64]
@@ -133,7 +138,8 @@ def lib():
         getattr(L, name).restype = C.c_int
     for name in ("ns_config_parse", "ns_create", "ns_init", "ns_set_initial_field", "ns_step", "ns_run", "ns_get_time",
                  "ns_get_field", "ns_get_node_types", "ns_get_counts", "ns_get_ghosts", "ns_get_filter_nodes",
-                 "ns_norm_history", "ns_force_history", "ns_totals", "ns_get_geometry", "ns_profile", "ns_set_option", "ns_get_tile"):
+                 "ns_norm_history", "ns_force_history", "ns_set_dt_schedule", "ns_set_inflow_profile",
+                 "ns_adjoint_drag", "ns_totals", "ns_get_geometry", "ns_profile", "ns_set_option", "ns_get_tile"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -376,6 +382,44 @@ class Solver:
         if cnt.value:
             self._check(lib().ns_force_history(self._h, _dptr(a), cnt.value, C.byref(cnt)))
         return a
+
+    def setDtSchedule(self, dts):
+        """Prescribed step sizes by time level (None: back to the stability limit)."""
+        if dts is None:
+            self._check(lib().ns_set_dt_schedule(self._h, None, 0))
+            return
+        a = np.ascontiguousarray(dts, dtype=np.float64)
+        self._check(lib().ns_set_dt_schedule(self._h, _dptr(a), a.size))
+
+    def inletPlaneShape(self):
+        """(n_b, n_a) shape of the inlet plane's profile array (x inlet: (NK, NJ))."""
+        c = self.config
+        axis = [c.bc[0], c.bc[1], c.bc[2], c.bc[3], c.bc[4], c.bc[5]].index(BC["inlet"]) // 2
+        return {0: (c.NK, c.NJ), 1: (c.NK, c.NI), 2: (c.NJ, c.NI)}[axis]
+
+    def setInflowProfile(self, q):
+        """Inlet velocity factors per inlet-plane node (None: uniform inlet)."""
+        if q is None:
+            self._check(lib().ns_set_inflow_profile(self._h, None, 0))
+            return
+        a = np.ascontiguousarray(q, dtype=np.float64)
+        self._check(lib().ns_set_inflow_profile(self._h, _dptr(a), a.size))
+
+    def adjointDrag(self, nsteps: int, n0: int, n1: int, checkpoint: int = 20):
+        """J = mean C_D over time levels n0..n1 of an nsteps run from the current initial state, and its
+        gradients (ns_adjoint_drag): dict with J, grad_u0 (5, NK, NJ, NI), grad_inflow (inlet plane),
+        grad_re and the run's step sizes dts (nsteps + 1)."""
+        N = self.mesh.nNodesTotal
+        gu = np.zeros(5 * N)
+        shp = self.inletPlaneShape()
+        gq = np.zeros(int(np.prod(shp)))
+        J, gre = C.c_double(), C.c_double()
+        dts = np.zeros(nsteps + 1)
+        self._check(lib().ns_adjoint_drag(self._h, int(nsteps), int(n0), int(n1), int(checkpoint), C.byref(J),
+                                          _dptr(gu), _dptr(gq), C.byref(gre), _dptr(dts)))
+        c = self.config
+        return {"J": J.value, "grad_u0": gu.reshape(5, c.NK, c.NJ, c.NI), "grad_inflow": gq.reshape(shp),
+                "grad_re": gre.value, "dts": dts}
 
     def referenceArea(self) -> float:
         """Frontal area for force coefficients: pi r^2 (sphere), 2 r times the span of FluidInterior planes

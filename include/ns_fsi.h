@@ -171,6 +171,31 @@ ns_status ns_norm_history(ns_handle* h, int32_t var, double* out, int64_t n, int
  * the number available. */
 ns_status ns_force_history(ns_handle* h, double* out, int64_t n, int64_t* count);
 
+/* Prescribed step sizes (no reference counterpart): dts[L] is the size of the step that leaves time level
+ * L; the stability limit is still scanned but not used. n must exceed the last time level stepped to.
+ * Null / n = 0 returns to the stability limit. Used to differentiate runs at a frozen step sequence. */
+ns_status ns_set_dt_schedule(ns_handle* h, const double* dts, int64_t n);
+
+/* Inlet velocity profile (extension; the reference's inlet is uniform): the inlet's normal velocity at a
+ * node of the inlet plane is min(1, t / inletRampTime) * M_0 * q[a + n_a * b], where (a, b) are the node's
+ * two in-plane indices in (i, j, k) order and n_a the extent of the first (x inlet: q[j + NJ * k]).
+ * n must equal the plane's node count. Null restores the uniform inlet. */
+ns_status ns_set_inflow_profile(ns_handle* h, const double* q, int64_t n);
+
+/* Discrete adjoint of the drag objective (no reference counterpart). From the initial state (time level 0,
+ * precision fp64, no periodic density filter, an immersed body), runs nsteps FP64 steps and evaluates
+ *   J = 1/(n1 - n0 + 1) * sum_{L=n0..n1} C_D(U^L),  C_D = Fx / (0.5 M_0^2 A_ref)
+ * (Fx and A_ref as in ns_force_history), then the hand-written reverse sweep (transposed right-hand side,
+ * combine and boundary application of every stage; checkpoints every `checkpoint` steps with
+ * recomputation) at the step sizes of that run held fixed. Outputs (any may be null):
+ *   J; grad_u0 = dJ/d(initial conserved fields), 5 x-fastest arrays as ns_get_field;
+ *   grad_inflow = dJ/d(inlet profile factors q), the inlet plane's size (see ns_set_inflow_profile);
+ *   grad_re = dJ/dRe; dts = the nsteps + 1 step sizes by time level (pass them to ns_set_dt_schedule to
+ *   evaluate J at the same frozen steps, e.g. for finite differences).
+ * The handle is left at time level nsteps with its previous step-size rule. */
+ns_status ns_adjoint_drag(ns_handle* h, int32_t nsteps, int32_t n0, int32_t n1, int32_t checkpoint, double* J,
+                          double* grad_u0, double* grad_inflow, double* grad_re, double* dts);
+
 /* FP64 totals over FluidInterior nodes of the current state: sum of each conserved variable times
  * dx*dy*dz (mass, momenta, energy). No reference counterpart; conservation monitor. */
 ns_status ns_totals(ns_handle* h, double out[5]);

@@ -1,0 +1,133 @@
+"""Discrete adjoint of the drag objective (ns_adjoint_drag) against central finite differences.
+
+J = mean C_D over time levels n0..n1; the adjoint differentiates the run at its own step sizes, so every
+finite-difference evaluation runs at the same prescribed step sequence (ns_set_dt_schedule). Directions
+are random over all conserved fields of all nodes of the initial state, over the inlet profile, and Re.
+Central differences with eps = 1e-4 (FP64) have a truncation error far below the 1e-6 relative bar
+used here at these sizes (the north-star bar is 1e-4).
+"""
+import numpy as np
+import pytest
+import torch
+
+import cases
+import nsfsi
+import nsfsi_torch as nt
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-4
+TOL = 1e-6
+
+
+def small_case(N=24):
+    return cases.sphere48(N=N, steps=10)  # FP64, sphere r=0.5 in [-2,2]^3, inlet ramp, symmetry walls
+
+
+def base(cfg):
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    U = s.mesh.conserved()
+    q = np.ones(s.inletPlaneShape())
+    s.close()
+    return U, q
+
+
+def J_at(cfg, U, q, re, nsteps, n0, n1, dts):
+    return nt.drag_and_gradients(cfg, U, q, re, nsteps, n0, n1, checkpoint=nsteps, dts=dts)["J"]
+
+
+def check_u0(nsteps, n0, n1, ckpt, ndir=3, N=24):
+    cfg = small_case(N)
+    U, q = base(cfg)
+    print(f"adjoint N={N} nsteps={nsteps} window={n0}..{n1} checkpoint={ckpt}", flush=True)
+    r = nt.drag_and_gradients(cfg, U, q, cfg.Re, nsteps, n0, n1, ckpt)
+    assert np.all(np.isfinite(r["grad_u0"]))
+    rng = np.random.default_rng(nsteps * 100 + n0)
+    worst = 0.0
+    for _ in range(ndir):
+        d = rng.standard_normal(U.shape)
+        d /= np.abs(d).max()
+        fd = (J_at(cfg, U + EPS * d, q, cfg.Re, nsteps, n0, n1, r["dts"]) -
+              J_at(cfg, U - EPS * d, q, cfg.Re, nsteps, n0, n1, r["dts"])) / (2 * EPS)
+        ad = float(np.sum(r["grad_u0"] * d))
+        rel = abs(fd - ad) / max(abs(fd), 1e-30)
+        print(f"  J={r['J']:.10e} FD={fd:.10e} adjoint={ad:.10e} rel={rel:.2e}", flush=True)
+        worst = max(worst, rel)
+    assert worst <= TOL, worst
+    return r
+
+
+def test_objective_at_initial_state():
+    """n0 = n1 = 0: only the force of the initial state (the transposed face fluxes)."""
+    check_u0(1, 0, 0, 1)
+
+
+def test_one_step():
+    check_u0(1, 1, 1, 1)
+
+
+def test_several_steps_two_segments():
+    check_u0(7, 3, 7, 4)
+
+
+def test_inflow_gradient():
+    """A short inlet ramp and enough steps for the inlet to reach the body; a coherent profile direction
+    (the profile scales the inlet velocity, so a random-sign direction mostly cancels)."""
+    cfg = small_case()
+    cfg.inlet_ramp_time = 0.05
+    U, q = base(cfg)
+    nsteps, n0, n1 = 24, 16, 24
+    r = nt.drag_and_gradients(cfg, U, q, cfg.Re, nsteps, n0, n1, 8)
+    rng = np.random.default_rng(7)
+    dq = 1.0 + 0.5 * rng.standard_normal(q.shape)
+    dq /= np.abs(dq).max()
+    fd = (J_at(cfg, U, q + EPS * dq, cfg.Re, nsteps, n0, n1, r["dts"]) -
+          J_at(cfg, U, q - EPS * dq, cfg.Re, nsteps, n0, n1, r["dts"])) / (2 * EPS)
+    ad = float(np.sum(r["grad_inflow"] * dq))
+    print(f"inflow J={r['J']:.10e} FD={fd:.10e} adjoint={ad:.10e}", flush=True)
+    assert abs(fd - ad) <= TOL * abs(fd)
+
+
+def test_reynolds_gradient():
+    cfg = small_case()
+    U, q = base(cfg)
+    nsteps, n0, n1 = 8, 4, 8
+    r = nt.drag_and_gradients(cfg, U, q, cfg.Re, nsteps, n0, n1, 3)
+    h = EPS * cfg.Re
+    fd = (J_at(cfg, U, q, cfg.Re + h, nsteps, n0, n1, r["dts"]) -
+          J_at(cfg, U, q, cfg.Re - h, nsteps, n0, n1, r["dts"])) / (2 * h)
+    print(f"Re FD={fd:.10e} adjoint={r['grad_re']:.10e}", flush=True)
+    assert abs(fd - r["grad_re"]) <= TOL * abs(fd)
+
+
+def test_torch_function_backward():
+    cfg = small_case()
+    U, q = base(cfg)
+    u0 = torch.tensor(U, dtype=torch.float64, requires_grad=True)
+    qq = torch.tensor(q, dtype=torch.float64, requires_grad=True)
+    re = torch.tensor(cfg.Re, dtype=torch.float64, requires_grad=True)
+    J = nt.DragObjective.apply(u0, qq, re, cfg, 4, 2, 4, 2)
+    (2.0 * J).backward()
+    r = nt.drag_and_gradients(cfg, U, q, cfg.Re, 4, 2, 4, 2)
+    assert J.item() == r["J"]
+    assert np.allclose(u0.grad.numpy(), 2.0 * r["grad_u0"], rtol=1e-12, atol=0)
+    assert np.allclose(qq.grad.numpy(), 2.0 * r["grad_inflow"], rtol=1e-12, atol=0)
+    assert re.grad.item() == pytest.approx(2.0 * r["grad_re"], rel=1e-12)
+
+
+def test_adjoint_leaves_final_state():
+    """After the run the handle holds the forward run's final state (as ns_step would)."""
+    cfg = small_case()
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    r = s.adjointDrag(5, 3, 5, 2)
+    U = s.mesh.conserved()
+    assert s.time()[1] == 5
+    s.close()
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    s.setDtSchedule(r["dts"])
+    s.marchTimeStep(5)
+    assert np.array_equal(s.mesh.conserved(), U)
+    s.close()
