@@ -1,0 +1,115 @@
+"""Adjoint study: gradients of the time-averaged drag coefficient against central finite differences.
+
+    python 3d-ns-fsi_amd/adjoint_study.py [--N 96] [--steps 400] [--n0 300] [--n1 400] [--checkpoint 20]
+                                          [--directions 16] [--eps 1e-4] [--out FILE]
+
+FP64 sphere case on [-2,2]^3 (r = 0.5 at the centre, Re_D = 100, M = 0.3, Gaussian perturbation 1e-3,
+seed 5). One forward + reverse run (ns_adjoint_drag) gives J and dJ/d(initial state), dJ/d(inlet profile)
+and dJ/dRe; each finite difference runs the forward solver twice at the same frozen step sequence
+(J = mean Fx / (0.5 M^2 pi r^2) over the window, from the recorded body force). Directions: random
+normal over all conserved fields of all nodes (max-norm 1), a coherent inlet-profile direction, and Re.
+"""
+import argparse
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+import cases  # noqa: E402
+import nsfsi  # noqa: E402
+import nsfsi_torch as nt  # noqa: E402
+
+
+def forward_J(cfg, U, q, re, nsteps, n0, n1, dts):
+    c = nt._config_with_re(cfg, re)
+    s = nsfsi.Solver(c)
+    try:
+        s.initialize()
+        s.setInitialField(U.reshape(-1))
+        s.setInflowProfile(q.reshape(-1))
+        s.setDtSchedule(dts)
+        s.setOption("body_force", 1)
+        s.setOption("graph", 1)
+        s.marchTimeStep(nsteps)
+        f = s.getForceHistory()
+        sel = (f[:, 0] >= n0) & (f[:, 0] <= n1)
+        assert sel.sum() == n1 - n0 + 1
+        return float(f[sel, 1].sum() / (n1 - n0 + 1) / (0.5 * c.M_0 ** 2 * s.referenceArea()))
+    finally:
+        s.close()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--N", type=int, default=96)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--n0", type=int, default=300)
+    ap.add_argument("--n1", type=int, default=400)
+    ap.add_argument("--checkpoint", type=int, default=20)
+    ap.add_argument("--directions", type=int, default=16)
+    ap.add_argument("--eps", type=float, default=1e-4)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    assert a.n0 >= 1, "the forward-only evaluation records forces after steps (n0 >= 1)"
+    cfg = cases.sphere96(N=a.N, steps=a.steps)
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    U = s.mesh.conserved()
+    q = np.ones(s.inletPlaneShape())
+    s.close()
+    t0 = time.perf_counter()
+    r = nt.drag_and_gradients(cfg, U, q, cfg.Re, a.steps, a.n0, a.n1, a.checkpoint)
+    t_adj = time.perf_counter() - t0
+    print(f"J = {r['J']:.12e}  (forward + reverse {t_adj:.1f} s)", flush=True)
+    t0 = time.perf_counter()
+    Jf = forward_J(cfg, U, q, cfg.Re, a.steps, a.n0, a.n1, r["dts"])
+    t_fwd = time.perf_counter() - t0
+    print(f"forward-only J = {Jf:.12e} ({t_fwd:.1f} s)", flush=True)
+    rows = []
+    rng = np.random.default_rng(2026)
+    for d in range(a.directions):
+        dU = rng.standard_normal(U.shape)
+        dU /= np.abs(dU).max()
+        fd = (forward_J(cfg, U + a.eps * dU, q, cfg.Re, a.steps, a.n0, a.n1, r["dts"]) -
+              forward_J(cfg, U - a.eps * dU, q, cfg.Re, a.steps, a.n0, a.n1, r["dts"])) / (2 * a.eps)
+        ad = float(np.sum(r["grad_u0"] * dU))
+        rows.append((f"initial state #{d + 1}", fd, ad))
+        print(rows[-1], flush=True)
+    dq = 1.0 + 0.5 * rng.standard_normal(q.shape)
+    dq /= np.abs(dq).max()
+    fd = (forward_J(cfg, U, q + a.eps * dq, cfg.Re, a.steps, a.n0, a.n1, r["dts"]) -
+          forward_J(cfg, U, q - a.eps * dq, cfg.Re, a.steps, a.n0, a.n1, r["dts"])) / (2 * a.eps)
+    rows.append(("inlet profile", fd, float(np.sum(r["grad_inflow"] * dq))))
+    h = a.eps * cfg.Re
+    fd = (forward_J(cfg, U, q, cfg.Re + h, a.steps, a.n0, a.n1, r["dts"]) -
+          forward_J(cfg, U, q, cfg.Re - h, a.steps, a.n0, a.n1, r["dts"])) / (2 * h)
+    rows.append(("Re", fd, r["grad_re"]))
+    lines = [f"# Adjoint study: sphere {a.N}^3, {a.steps} FP64 RK4 steps, J = mean C_D over levels {a.n0}-{a.n1}", "",
+             f"Command (repository root, one MI355X): `python 3d-ns-fsi_amd/adjoint_study.py --N {a.N} --steps {a.steps} "
+             f"--n0 {a.n0} --n1 {a.n1} --checkpoint {a.checkpoint} --directions {a.directions} --eps {a.eps}`", "",
+             f"J = {r['J']:.12e} (adjoint run) / {Jf:.12e} (forward-only run at the same step sizes).",
+             f"Wall time: forward + checkpointed reverse {t_adj:.1f} s (checkpoint every {a.checkpoint} steps); "
+             f"one forward run {t_fwd:.1f} s.", "",
+             f"Central differences, eps = {a.eps} (FP64):", "",
+             "| direction | finite difference | adjoint | relative difference |", "|---|---|---|---|"]
+    worst = 0.0
+    for name, fd, ad in rows:
+        rel = abs(fd - ad) / max(abs(fd), 1e-300)
+        worst = max(worst, rel)
+        lines.append(f"| {name} | {fd:.10e} | {ad:.10e} | {rel:.2e} |")
+    lines += ["", f"Largest relative difference: {worst:.2e} (bar: 1e-4)."]
+    text = "\n".join(lines) + "\n"
+    print(text)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(text)
+    if not math.isfinite(worst) or worst > 1e-4:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()
