@@ -101,22 +101,23 @@ struct FusedArgs {
 
 // Whole-wavefront shifts by one lane (DPP wave_shl:1 / wave_shr:1 of GFX9): shl1 gives each lane the
 // value of lane + 1, shr1 the value of lane - 1 (the edge lane receives 0; edge lanes are halo lanes).
+// bound_ctrl set: the lane without a source reads 0 in hardware (no move of an old value first).
 __device__ __forceinline__ float shl1(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float shr1(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ double shl1(double x) {
     int2 v = *reinterpret_cast<int2*>(&x);
-    v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x130, 0xf, 0xf, false);
-    v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x130, 0xf, 0xf, false);
+    v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x130, 0xf, 0xf, true);
+    v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x130, 0xf, 0xf, true);
     return *reinterpret_cast<double*>(&v);
 }
 __device__ __forceinline__ double shr1(double x) {
     int2 v = *reinterpret_cast<int2*>(&x);
-    v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x138, 0xf, 0xf, false);
-    v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x138, 0xf, 0xf, false);
+    v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x138, 0xf, 0xf, true);
+    v.y = __builtin_amdgcn_update_dpp(0, v.y, 0x138, 0xf, 0xf, true);
     return *reinterpret_cast<double*>(&v);
 }
 
@@ -128,6 +129,62 @@ template <class T>
 __device__ __forceinline__ void stAt(T* base, uint32_t boff, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
+
+// Field accessors of one stage launch. FP32: buffer instructions, one resource per 5-field state buffer
+// (<= 4 GiB), the node's byte offset in a VGPR and the field's in an SGPR, so no address arithmetic per
+// access. FP64 (a 5-field buffer can exceed 4 GiB): global accesses at per-field base pointers.
+template <class T>
+struct Mem;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+
+template <>
+struct Mem<float> {
+    __amdgpu_buffer_rsrc_t sin, u, a, sout, comp;
+    uint32_t fsb;  // field stride in bytes (FP32)
+    __device__ explicit Mem(const FusedArgs<float>& x) {
+        fsb = (uint32_t)(x.L.fstride * sizeof(float));
+        const uint32_t nb = 5 * fsb;
+        sin = rsrc(x.Sin[0], nb);
+        u = rsrc(x.U[0], nb);
+        a = rsrc(x.A[0], nb);
+        sout = rsrc(x.Sout[0], nb);
+        comp = rsrc(x.comp[0], nb / 2);
+    }
+    __device__ float ld(__amdgpu_buffer_rsrc_t r, int q, uint32_t bo) const {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)bo, (int)(q * fsb), 0));
+    }
+    __device__ void st(__amdgpu_buffer_rsrc_t r, int q, uint32_t bo, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)bo, (int)(q * fsb), 0);
+    }
+    __device__ float ldSin(int q, uint32_t bo) const { return ld(sin, q, bo); }
+    __device__ float ldU(int q, uint32_t bo) const { return ld(u, q, bo); }
+    __device__ float ldA(int q, uint32_t bo) const { return ld(a, q, bo); }
+    __device__ void stA(int q, uint32_t bo, float v) const { st(a, q, bo, v); }
+    __device__ void stSout(int q, uint32_t bo, float v) const { st(sout, q, bo, v); }
+    // compensation: bfloat16 fields at half the byte offsets
+    __device__ uint16_t ldComp(int q, uint32_t bo) const {
+        return __builtin_amdgcn_raw_buffer_load_b16(comp, (int)(bo >> 1), (int)(q * (fsb >> 1)), 0);
+    }
+    __device__ void stComp(int q, uint32_t bo, uint16_t v) const {
+        __builtin_amdgcn_raw_buffer_store_b16(v, comp, (int)(bo >> 1), (int)(q * (fsb >> 1)), 0);
+    }
+};
+
+template <>
+struct Mem<double> {
+    const FusedArgs<double>& x;
+    __device__ explicit Mem(const FusedArgs<double>& a) : x(a) {}
+    __device__ double ldSin(int q, uint32_t bo) const { return ldAt(x.Sin[q], bo); }
+    __device__ double ldU(int q, uint32_t bo) const { return ldAt(x.U[q], bo); }
+    __device__ double ldA(int q, uint32_t bo) const { return ldAt((const double*)x.A[q], bo); }
+    __device__ void stA(int q, uint32_t bo, double v) const { stAt(x.A[q], bo, v); }
+    __device__ void stSout(int q, uint32_t bo, double v) const { stAt(x.Sout[q], bo, v); }
+    __device__ uint16_t ldComp(int, uint32_t) const { return 0; }
+    __device__ void stComp(int, uint32_t, uint16_t) const {}
+};
 
 // Point state at one plane of the own column.
 template <class T>
@@ -222,17 +279,17 @@ struct Raw {
 };
 
 template <class T>
-__device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, int i, int j, int k) {
+__device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, const Mem<T>& m, int i, int j, int k) {
     const Layout& L = a.L;
     const int ic = min(max(i, 0), L.NI - 1), jc = min(max(j, 0), L.NJ - 1), kc = min(max(k, 0), L.NK - 1);
     const size_t o = L.off(ic, jc, kc);
     const uint32_t bo = (uint32_t)(o * sizeof(T));
     Raw<T> r;
-    r.rho = ldAt(a.Sin[0], bo);
-    r.ru = ldAt(a.Sin[1], bo);
-    r.rv = ldAt(a.Sin[2], bo);
-    r.rw = ldAt(a.Sin[3], bo);
-    r.E = ldAt(a.Sin[4], bo);
+    r.rho = m.ldSin(0, bo);
+    r.ru = m.ldSin(1, bo);
+    r.rv = m.ldSin(2, bo);
+    r.rw = m.ldSin(3, bo);
+    r.E = m.ldSin(4, bo);
     r.rp = r.rho;
     r.Ep = r.E;
     if (a.W.contains(ic, jc, kc) && (a.cls[o] & CLS_FILTER)) {
@@ -300,16 +357,17 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     const double dtd = *a.dtdev;
     const bool rhoFiltered = STAGE == 1 && a.fFlag && *a.fFlag;
     static_assert(!KAHAN || sizeof(T) == 4, "the compensated combine is the FP32 form");
+    const Mem<T> m(a);
 
     // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
     Pt<T> qm, q0, q1;
-    Raw<T> rnext = fetch(a, i, j, kb - 1);
+    Raw<T> rnext = fetch(a, m, i, j, kb - 1);
     const bool inner = row >= 1 && row <= TY;  // output rows (halo rows are 0 and TY+1)
     finishPlane<T, TY>(a, sm, row, lane, rnext, qm, inner);
-    rnext = fetch(a, i, j, kb);
+    rnext = fetch(a, m, i, j, kb);
     __syncthreads();
     finishPlane<T, TY>(a, sm, row, lane, rnext, q0, inner);
-    rnext = fetch(a, i, j, kb + 1);
+    rnext = fetch(a, m, i, j, kb + 1);
     T gz0[5], phizm[5];
     if (inner) {
         T gzm[5], cz[5];
@@ -329,7 +387,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
 
     for (int k = kb; k < ke; ++k) {
         const Raw<T> rcur = rnext;
-        rnext = fetch(a, i, j, k + 2);
+        rnext = fetch(a, m, i, j, k + 2);
         // combine inputs of the output node, fetched before the stencil work
         const bool doOut = outRow && outLane;
         const size_t o = doOut ? L.off(i, j, k) : 0;
@@ -346,13 +404,13 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
             }
             if (STAGE == 2 || STAGE == 3 || (STAGE == 4 && KAHAN) || (SCAN && a.norms)) {
                 if (interior)
-                    for (int q = 0; q < 5; ++q) base[q] = ldAt(a.U[q], bo);
+                    for (int q = 0; q < 5; ++q) base[q] = m.ldU(q, bo);
             }
             if (STAGE >= 2) {
                 if (interior)
-                    for (int q = 0; q < 5; ++q) acc[q] = ldAt((const T*)a.A[q], bo);
+                    for (int q = 0; q < 5; ++q) acc[q] = m.ldA(q, bo);
             } else if (!interior) {
-                for (int q = 0; q < 5; ++q) acc[q] = ldAt((const T*)a.A[q], bo);
+                for (int q = 0; q < 5; ++q) acc[q] = m.ldA(q, bo);
             }
         }
         // sm.uvw is rewritten here: every reader of the previous plane's uvw has passed the face barrier
@@ -416,14 +474,14 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                         const T c0 = cur[0];
                         cur[0] = rhoFiltered ? ldAt(a.fRho, bo) : c0;
                         for (int q = 0; q < 5; ++q) {
-                            stAt(a.Sout[q], bo, cur[q] + h2 * rhs[q]);
-                            stAt(a.A[q], bo, KAHAN ? h6 * rhs[q] : cur[q] + h6 * rhs[q]);
+                            m.stSout(q, bo, cur[q] + h2 * rhs[q]);
+                            m.stA(q, bo, KAHAN ? h6 * rhs[q] : cur[q] + h6 * rhs[q]);
                         }
                     } else if (STAGE == 2 || STAGE == 3) {
                         const T hs = (T)(STAGE == 2 ? dtd / 2 : dtd), h3 = (T)(dtd / 3);
                         for (int q = 0; q < 5; ++q) {
-                            stAt(a.Sout[q], bo, base[q] + hs * rhs[q]);
-                            stAt(a.A[q], bo, acc[q] + h3 * rhs[q]);
+                            m.stSout(q, bo, base[q] + hs * rhs[q]);
+                            m.stA(q, bo, acc[q] + h3 * rhs[q]);
                         }
                     } else {
                         const T h6 = (T)(dtd / 6);
@@ -432,13 +490,11 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                             an[q] = acc[q] + h6 * rhs[q];
                             if constexpr (KAHAN) {
                                 // U^{n+1} = U^n + (D + e^{n-1}), its rounding error kept for the next step
-                                const uint32_t ho = bo >> 1;
-                                uint16_t* cq = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(a.comp[q]) + ho);
                                 T e;
-                                an[q] = twoSum(base[q], an[q] + bf16Decode(*cq), e);
-                                *cq = bf16Encode(e);
+                                an[q] = twoSum(base[q], an[q] + bf16Decode(m.ldComp(q, bo)), e);
+                                m.stComp(q, bo, bf16Encode(e));
                             }
-                            stAt(a.A[q], bo, an[q]);
+                            m.stA(q, bo, an[q]);
                         }
                         // end-of-step scans; filter nodes change once more in the boundary pass and are
                         // scanned there (src/Solver.cpp:91-126, src/Mesh.cpp:171-178, src/Mesh.h:48-61)
@@ -464,12 +520,12 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                         // the reference's target array holds the unfiltered density of U after the
                         // filter's buffer swap (src/Mesh.cpp:133), the other fields of the step before
                         if (rhoFiltered) acc[0] = cur[0];
-                        for (int q = 0; q < 5; ++q) stAt(a.Sout[q], bo, acc[q]);
+                        for (int q = 0; q < 5; ++q) m.stSout(q, bo, acc[q]);
                     }
                     else if (STAGE == 4)
-                        for (int q = 0; q < 5; ++q) stAt(a.A[q], bo, cur[q]);
+                        for (int q = 0; q < 5; ++q) m.stA(q, bo, cur[q]);
                     else
-                        for (int q = 0; q < 5; ++q) stAt(a.Sout[q], bo, cur[q]);
+                        for (int q = 0; q < 5; ++q) m.stSout(q, bo, cur[q]);
                 }
             }
         }
