@@ -19,6 +19,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "3d-ns-fsi_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per stage launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy), per
+# workload configuration; a run whose configuration differs reports traffic as null.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_fused_fp32_sphere512_traffic.json")
 VALUES_PER_NODE_STEP = 80  # 3-register RK4: stage 1 reads 5 writes 10, stages 2-3 read 15 write 10, stage 4 reads 10 writes 5
 
 
@@ -44,6 +47,19 @@ def cpu_baseline(workload, precision_unused=None):
         "sample": f"{workload} geometry and physics at {N}^3, {steps} RK4 steps, serial FP64 oracle "
                   f"(g++ -O3, no FMA), {el:.1f} s; host has {os.cpu_count()} cores",
     }
+
+
+def measured_traffic(workload, precision, kahan, tile, kernel):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (kernel == "fused" and t.get("workload") == workload and t.get("precision") == precision
+            and bool(t.get("compensated_combine")) == bool(kahan) and tile is not None
+            and list(t.get("tile", [])) == list(tile)):
+        return t.get("traffic_bytes_per_stage_launch")
+    return None
 
 
 def copy_calibration(nbytes):
@@ -198,6 +214,9 @@ def main():
     alg_bytes = VALUES_PER_NODE_STEP / 4 * eb * nodes  # per stage launch
     achieved = alg_bytes / (ms[0] * 1e-3) / 1e9 if ms[0] > 0 else None
     g, f, _ = s.mesh.counts()
+    kahan = bool(prec == "fp32" and cfg.kahan)
+    traffic = measured_traffic(a.workload, prec, kahan, tile, a.kernel)
+    traffic = round(traffic) if traffic else None
     line = {
         "metric": "M node-updates/s per RK4 step",
         "value": round(value, 3),
@@ -212,7 +231,7 @@ def main():
         "dtype": "f32" if prec == "fp32" else "f64",
         "data": "synthetic (seeded freestream + perturbation initial field)",
         "config": {"workload": a.workload, "grid": [cfg.NI, cfg.NJ, cfg.NK], "precision": prec,
-                   "compensated_combine": bool(prec == "fp32" and cfg.kahan),
+                   "compensated_combine": kahan,
                    "ghosts": g, "filter_nodes": f, "stage_kernel": a.kernel,
                    "tile_rows_x_zplanes": list(tile) if tile else None,
                    "hip_graph": not a.no_graph,
@@ -223,7 +242,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "k_stage_fused (RK4 stage)" if a.kernel == "fused" else "k_rhs_plain (RK4 stage)",
             "kernel_ms": round(ms[0], 4),
             "step_breakdown_ms": {"stage_kernel": round(ms[0] * launches[0], 3),
