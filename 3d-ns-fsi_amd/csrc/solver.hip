@@ -532,6 +532,8 @@ void Case::setOption(const std::string& name, int64_t value) {
         // the partial-slot buffer of the in-kernel scans is sized for z-chunks of >= kMinTileKc planes
         if (value < kMinTileKc || value > 4096) throw std::invalid_argument("tile_kc out of range (8..4096)");
         tile.kc = (int)value;
+    } else if (name == "global_access") {
+        tile.globalAccess = value != 0;
     } else if (name == "autotune") {
         if (!initialized) throw StateError("autotune before init");
         autotune(value > 0 ? (int)value : 3);
@@ -557,6 +559,7 @@ double Case::autotune(int reps) {
             FusedTile tl;
             tl.ty = tys[a];
             tl.kc = kcs[b];
+            tl.globalAccess = tile.globalAccess;
             auto run = [&](int count) {
                 for (int r = 0; r < count; ++r) {
                     if (fp32)

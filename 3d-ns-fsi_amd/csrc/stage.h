@@ -55,6 +55,9 @@ void launchRhoCommit(hipStream_t s, const Layout& L, T* U, const T* filtered, co
 struct FusedTile {
     int ty = 8;
     int kc = 64;
+    // FP32: global accesses at per-field base pointers even when the state fits one buffer resource
+    // (the form every FP32 state beyond 4 GiB uses; same arithmetic, so bit-identical results)
+    bool globalAccess = false;
 };
 constexpr int kMinTileKc = 8;  // smallest z-chunk accepted (sizes the scan partial buffers)
 // Stage 4 with partials != null also writes one partial per block of the end-of-step scans over

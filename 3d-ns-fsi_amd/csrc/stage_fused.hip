@@ -130,10 +130,11 @@ __device__ __forceinline__ void stAt(T* base, uint32_t boff, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
 
-// Field accessors of one stage launch. FP32: buffer instructions, one resource per 5-field state buffer
-// (<= 4 GiB), the node's byte offset in a VGPR and the field's in an SGPR, so no address arithmetic per
-// access. FP64 (a 5-field buffer can exceed 4 GiB): global accesses at per-field base pointers.
-template <class T>
+// Field accessors of one stage launch. BUF (FP32 states of at most 4 GiB): buffer instructions, one
+// resource per 5-field state buffer, the node's byte offset in a VGPR and the field's in an SGPR, so no
+// address arithmetic per access. Otherwise (FP64, or an FP32 state beyond 4 GiB): global accesses at
+// per-field base pointers with the node's 32-bit byte offset.
+template <class T, bool BUF>
 struct Mem;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, uint32_t bytes) {
@@ -141,7 +142,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, uint32_t b
 }
 
 template <>
-struct Mem<float> {
+struct Mem<float, true> {
     __amdgpu_buffer_rsrc_t sin, u, a, sout, comp;
     uint32_t fsb;  // field stride in bytes (FP32)
     __device__ explicit Mem(const FusedArgs<float>& x) {
@@ -173,18 +174,26 @@ struct Mem<float> {
     }
 };
 
-template <>
-struct Mem<double> {
-    const FusedArgs<double>& x;
-    __device__ explicit Mem(const FusedArgs<double>& a) : x(a) {}
-    __device__ double ldSin(int q, uint32_t bo) const { return ldAt(x.Sin[q], bo); }
-    __device__ double ldU(int q, uint32_t bo) const { return ldAt(x.U[q], bo); }
-    __device__ double ldA(int q, uint32_t bo) const { return ldAt((const double*)x.A[q], bo); }
-    __device__ void stA(int q, uint32_t bo, double v) const { stAt(x.A[q], bo, v); }
-    __device__ void stSout(int q, uint32_t bo, double v) const { stAt(x.Sout[q], bo, v); }
-    __device__ uint16_t ldComp(int, uint32_t) const { return 0; }
-    __device__ void stComp(int, uint32_t, uint16_t) const {}
+template <class T>
+struct Mem<T, false> {
+    const FusedArgs<T>& x;
+    __device__ explicit Mem(const FusedArgs<T>& a) : x(a) {}
+    __device__ T ldSin(int q, uint32_t bo) const { return ldAt(x.Sin[q], bo); }
+    __device__ T ldU(int q, uint32_t bo) const { return ldAt(x.U[q], bo); }
+    __device__ T ldA(int q, uint32_t bo) const { return ldAt((const T*)x.A[q], bo); }
+    __device__ void stA(int q, uint32_t bo, T v) const { stAt(x.A[q], bo, v); }
+    __device__ void stSout(int q, uint32_t bo, T v) const { stAt(x.Sout[q], bo, v); }
+    __device__ uint16_t ldComp(int q, uint32_t bo) const {
+        if constexpr (sizeof(T) == 4) return ldAt((const uint16_t*)x.comp[q], bo >> 1);
+        else return 0;
+    }
+    __device__ void stComp(int q, uint32_t bo, uint16_t v) const {
+        if constexpr (sizeof(T) == 4) stAt(x.comp[q], bo >> 1, v);
+    }
 };
+
+// Whether an FP32 state buffer (five fields) fits one buffer resource.
+inline bool stateFitsBuffer(const Layout& L) { return 5 * L.fstride * sizeof(float) <= 0xFFFFFFFFull; }
 
 // Point state at one plane of the own column.
 template <class T>
@@ -271,29 +280,39 @@ struct Smem {
     T face[10][TY + 2][64];
 };
 
-// Raw conserved values of one point, fetched ahead of use (software pipelining).
+// Raw conserved values of one point, fetched ahead of use (software pipelining). The pre-filter density
+// and energy of a filter node are kept beside the raw values and selected at use, so no register copy of
+// a load in flight is needed at fetch time (that copy would make the wave wait for the load there).
 template <class T>
 struct Raw {
     T rho, ru, rv, rw, E;  // current buffer values
-    T rp, Ep;              // density and energy for the primitives (pre-filter at filter nodes)
+    T rp, Ep;              // pre-filter density and energy (filter nodes only)
+    bool filt;
 };
 
+// Byte offset of node (i, j, k): 32-bit in FP32 (a state buffer of five fields stays below 4 GiB), the
+// column part per thread and the plane part uniform.
 template <class T>
-__device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, const Mem<T>& m, int i, int j, int k) {
+__device__ __forceinline__ uint32_t colBytes(const Layout& L, int i, int j) {
+    return (uint32_t)(j * L.pitch + i) * (uint32_t)sizeof(T);
+}
+
+template <class T, class M>
+__device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, const M& m, int ic, int jc, uint32_t col, int k) {
     const Layout& L = a.L;
-    const int ic = min(max(i, 0), L.NI - 1), jc = min(max(j, 0), L.NJ - 1), kc = min(max(k, 0), L.NK - 1);
-    const size_t o = L.off(ic, jc, kc);
-    const uint32_t bo = (uint32_t)(o * sizeof(T));
+    const int kc = min(max(k, 0), L.NK - 1);
+    const uint32_t bo = col + (uint32_t)kc * (uint32_t)(L.plane * sizeof(T));
     Raw<T> r;
     r.rho = m.ldSin(0, bo);
     r.ru = m.ldSin(1, bo);
     r.rv = m.ldSin(2, bo);
     r.rw = m.ldSin(3, bo);
     r.E = m.ldSin(4, bo);
-    r.rp = r.rho;
-    r.Ep = r.E;
-    if (a.W.contains(ic, jc, kc) && (a.cls[o] & CLS_FILTER)) {
+    r.rp = r.Ep = 0;
+    r.filt = false;
+    if (a.W.contains(ic, jc, kc) && (a.cls[L.off(ic, jc, kc)] & CLS_FILTER)) {
         size_t w = a.W.idx(ic, jc, kc);
+        r.filt = true;
         r.rp = a.rhoPre[w];
         r.Ep = a.rhoEPre[w];
     }
@@ -309,7 +328,7 @@ __device__ __forceinline__ void finishPlane(const FusedArgs<T>& a, Smem<T, TY>& 
     q.rv = r.rv;
     q.rw = r.rw;
     q.E = r.E;
-    prims(a.c, r.rp, r.ru, r.rv, r.rw, r.Ep, q);
+    prims(a.c, r.filt ? r.rp : r.rho, r.ru, r.rv, r.rw, r.filt ? r.Ep : r.E, q);
     T uE = shl1(q.u), uW = shr1(q.u), vE = shl1(q.v), vW = shr1(q.v), wE = shl1(q.w), wW = shr1(q.w);
     q.dxu = (uE - uW) * a.c.i2dx;
     q.dxv = (vE - vW) * a.c.i2dx;
@@ -340,7 +359,7 @@ __device__ __forceinline__ void readFaceNb(const Smem<T, TY>& sm, int r, int lan
 
 // One tile (block lb of the case described by a): the whole stage for 62 x TY output columns over one
 // z-chunk.
-template <class T, int STAGE, int TY, bool SCAN, bool KAHAN>
+template <class T, int STAGE, int TY, bool SCAN, bool KAHAN, bool BUF>
 __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, Smem<T, TY>& sm,
                                           double (&wred)[TY + 2][10]) {
     const int bx = lb % a.nbx;
@@ -357,17 +376,24 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     const double dtd = *a.dtdev;
     const bool rhoFiltered = STAGE == 1 && a.fFlag && *a.fFlag;
     static_assert(!KAHAN || sizeof(T) == 4, "the compensated combine is the FP32 form");
-    const Mem<T> m(a);
+    static_assert(!BUF || sizeof(T) == 4, "buffer-resource access is the FP32 form");
+    const Mem<T, BUF> m(a);
+    // clamped column of the stencil point (halo lanes and rows beyond the grid repeat the edge) and the
+    // own column's byte offset
+    const int ic = min(max(i, 0), L.NI - 1), jc = min(max(j, 0), L.NJ - 1);
+    const uint32_t colc = colBytes<T>(L, ic, jc);
+    const uint32_t colo = colBytes<T>(L, i, j);
+    const uint32_t planeB = (uint32_t)(L.plane * sizeof(T));
 
     // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
     Pt<T> qm, q0, q1;
-    Raw<T> rnext = fetch(a, m, i, j, kb - 1);
+    Raw<T> rnext = fetch(a, m, ic, jc, colc, kb - 1);
     const bool inner = row >= 1 && row <= TY;  // output rows (halo rows are 0 and TY+1)
     finishPlane<T, TY>(a, sm, row, lane, rnext, qm, inner);
-    rnext = fetch(a, m, i, j, kb);
+    rnext = fetch(a, m, ic, jc, colc, kb);
     __syncthreads();
     finishPlane<T, TY>(a, sm, row, lane, rnext, q0, inner);
-    rnext = fetch(a, m, i, j, kb + 1);
+    rnext = fetch(a, m, ic, jc, colc, kb + 1);
     T gz0[5], phizm[5];
     if (inner) {
         T gzm[5], cz[5];
@@ -387,11 +413,11 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
 
     for (int k = kb; k < ke; ++k) {
         const Raw<T> rcur = rnext;
-        rnext = fetch(a, m, i, j, k + 2);
+        rnext = fetch(a, m, ic, jc, colc, k + 2);
         // combine inputs of the output node, fetched before the stencil work
         const bool doOut = outRow && outLane;
-        const size_t o = doOut ? L.off(i, j, k) : 0;
-        const uint32_t bo = (uint32_t)(o * sizeof(T));
+        const uint32_t bo = doOut ? colo + (uint32_t)k * planeB : 0;
+        const size_t o = bo / sizeof(T);
         bool interior = false, filterNode = false;
         T base[5], acc[5];
         (void)filterNode;
@@ -557,7 +583,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
 }
 
 // FP32: at least 4 waves per SIMD (<= 128 VGPRs); FP64: at least 2.
-template <class T, int STAGE, int TY, bool SCAN, bool KAHAN>
+template <class T, int STAGE, int TY, bool SCAN, bool KAHAN, bool BUF>
 __global__ __launch_bounds__(64 * (TY + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void k_stage_fused(FusedArgs<T> a) {
     __shared__ Smem<T, TY> sm;
@@ -567,7 +593,7 @@ void k_stage_fused(FusedArgs<T> a) {
     const int per = (a.nblocks + 7) / 8;
     const int lb = (p % 8) * per + p / 8;
     if (lb >= a.nblocks) return;
-    stageTile<T, STAGE, TY, SCAN, KAHAN>(a, lb, sm, wred);
+    stageTile<T, STAGE, TY, SCAN, KAHAN, BUF>(a, lb, sm, wred);
 }
 
 // Many independent cases in one launch: the tiles of case c are the global tiles start[c] ..
@@ -588,7 +614,7 @@ void k_stage_batched(const FusedArgs<T>* __restrict__ tab, const int* __restrict
         if (start[mid] <= g) lo = mid;
         else hi = mid - 1;
     }
-    stageTile<T, STAGE, TY, SCAN, KAHAN>(tab[lo], g - start[lo], sm, wred);
+    stageTile<T, STAGE, TY, SCAN, KAHAN, sizeof(T) == 4>(tab[lo], g - start[lo], sm, wred);
 }
 
 // ---- force on the body: the discrete momentum balance of the flux form. Summed over all fluid interior
@@ -729,30 +755,29 @@ int fusedBlocks(const Layout& L, int kc) {
     return ((L.NI + 61) / 62) * ((L.NJ + TY - 1) / TY) * ((L.NK + kc - 1) / kc);
 }
 
-template <class T, int TY, bool K>
+template <class T, int TY, bool K, bool B>
 void launchTYK(hipStream_t s, int stage, const FusedArgs<T>& a) {
     const int grid = (a.nblocks + 7) / 8 * 8;
     dim3 blk(64, TY + 2);
     switch (stage) {
-        case 1: hipLaunchKernelGGL((k_stage_fused<T, 1, TY, false, K>), dim3(grid), blk, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_stage_fused<T, 2, TY, false, K>), dim3(grid), blk, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage_fused<T, 3, TY, false, K>), dim3(grid), blk, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_stage_fused<T, 1, TY, false, K, B>), dim3(grid), blk, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_stage_fused<T, 2, TY, false, K, B>), dim3(grid), blk, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage_fused<T, 3, TY, false, K, B>), dim3(grid), blk, 0, s, a); break;
         default:
-            if (a.part) hipLaunchKernelGGL((k_stage_fused<T, 4, TY, true, K>), dim3(grid), blk, 0, s, a);
-            else hipLaunchKernelGGL((k_stage_fused<T, 4, TY, false, K>), dim3(grid), blk, 0, s, a);
+            if (a.part) hipLaunchKernelGGL((k_stage_fused<T, 4, TY, true, K, B>), dim3(grid), blk, 0, s, a);
+            else hipLaunchKernelGGL((k_stage_fused<T, 4, TY, false, K, B>), dim3(grid), blk, 0, s, a);
             break;
     }
 }
 
 template <class T, int TY>
-void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a) {
+void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a, bool buf) {
     if constexpr (sizeof(T) == 4) {
-        if (a.comp[0]) {
-            launchTYK<T, TY, true>(s, stage, a);
-            return;
-        }
+        if (a.comp[0]) buf ? launchTYK<T, TY, true, true>(s, stage, a) : launchTYK<T, TY, true, false>(s, stage, a);
+        else buf ? launchTYK<T, TY, false, true>(s, stage, a) : launchTYK<T, TY, false, false>(s, stage, a);
+    } else {
+        launchTYK<T, TY, false, false>(s, stage, a);
     }
-    launchTYK<T, TY, false>(s, stage, a);
 }
 
 template <class T, int TY, bool K>
@@ -818,11 +843,12 @@ int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P,
     FusedArgs<T> a;
     fillArgs(a, stage, L, P, W, Sin, U, Sout, A, cls, rhoPre, rhoEPre, dtdev, tile, partials, norms, filteredRho,
              filterFlag, comp);
+    const bool buf = !tile.globalAccess && stateFitsBuffer(L);
     switch (normTY(tile.ty)) {
-        case 4: launchTY<T, 4>(s, stage, a); break;
-        case 6: launchTY<T, 6>(s, stage, a); break;
-        case 14: launchTY<T, 14>(s, stage, a); break;
-        default: launchTY<T, 8>(s, stage, a); break;
+        case 4: launchTY<T, 4>(s, stage, a, buf); break;
+        case 6: launchTY<T, 6>(s, stage, a, buf); break;
+        case 14: launchTY<T, 14>(s, stage, a, buf); break;
+        default: launchTY<T, 8>(s, stage, a, buf); break;
     }
     return a.nblocks;
 }
@@ -832,6 +858,7 @@ size_t fusedArgsBytes() { return sizeof(FusedArgs<float>); }
 int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const Window& W, const float* Sin,
                    const float* U, float* Sout, float* A, const uint8_t* cls, const float* rhoPre, const float* rhoEPre,
                    const double* dtdev, FusedTile tile, double* partials, int norms, uint16_t* comp) {
+    if (!stateFitsBuffer(L)) throw std::length_error("batched case: state buffer beyond 4 GiB");
     FusedArgs<float> a;
     fillArgs<float>(a, stage, L, P, W, Sin, U, Sout, A, cls, rhoPre, rhoEPre, dtdev, tile, partials, norms, nullptr,
                     nullptr, comp);

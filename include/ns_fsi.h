@@ -222,6 +222,9 @@ ns_status ns_profile(ns_handle* h, int32_t nsteps, double ms[4], int32_t launche
  *   "body_force"   1: record the force on the body after every step (ns_force_history), 0 (default);
  *   "graph"        1: ns_step replays each time step as one captured HIP graph (no per-kernel launch
  *                  cost), 0 (default): launches kernel by kernel. Same results either way.
+ *   "global_access" FP32 fused stage: 1 addresses the fields by per-field global pointers (the form
+ *                  used for states beyond 4 GiB, which one buffer resource cannot span), 0 (default)
+ *                  by buffer resources where the state fits. Same results either way.
  * Results do not depend on the tile shape (the per-node arithmetic is the same). */
 ns_status ns_set_option(ns_handle* h, const char* name, int64_t value);
 

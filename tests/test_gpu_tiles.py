@@ -26,7 +26,8 @@ def run(cfg, steps, **opts):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_tile_shape_invariance(prec):
-    cfg = cases.sphere48(N=40, steps=4, precision=prec)
+    # 41 planes: the last z-chunk of every tile height has an odd plane count (9 or 41)
+    cfg = cases.sphere48(N=41, steps=4, precision=prec)
     ref = run(cfg, 4, tile_ty=8, tile_kc=64)
     for ty in (4, 6, 14):
         for kc in (16, 32, 128):
@@ -44,3 +45,14 @@ def test_autotune_leaves_state_unchanged():
     assert ty in (4, 6, 8, 14) and kc in (16, 32, 64, 128)
     s.marchTimeStep(3)
     assert np.array_equal(s.mesh.conserved(), ref)
+
+
+@pytest.mark.parametrize("kahan", [1, 0])
+def test_fp32_global_access_bitwise(kahan):
+    """FP32 states beyond 4 GiB use per-field global addressing instead of buffer resources; forcing that
+    form on a small case gives the same state bit for bit (compensated and plain combine)."""
+    cfg = cases.sphere48(N=41, steps=6, precision="fp32")
+    cfg.kahan = kahan
+    ref = run(cfg, 6, tile_ty=6, tile_kc=16)
+    U = run(cfg, 6, tile_ty=6, tile_kc=16, global_access=1)
+    assert np.array_equal(U, ref)
