@@ -384,12 +384,12 @@ __global__ void k_adj_filter_give(Layout L, Window W, const int64_t* fn, int64_t
     const int i = (int)(m % L.NI), j = (int)((m / L.NI) % L.NJ), k = (int)(m / ((int64_t)L.NI * L.NJ));
     const size_t o = L.off(i, j, k);
     const size_t E4 = 4 * L.fstride;
-    const long sx = 1, sy = L.pitch, sz = (long)L.plane;
+    const long sy = L.pitch, sz = (long)L.plane;
     const double br = tmp[2 * q] / 12.0, be = tmp[2 * q + 1] / 12.0;
     const size_t w = W.idx(i, j, k);
     atomAdd(&Sb[o], 6.0 * br + Pb[w]);
     atomAdd(&Sb[o + E4], 6.0 * be + PEb[w]);
-    const long nbs[6] = {sx, -sx, sy, -sy, sz, -sz};
+    const long nbs[6] = {Layout::xstep(i, 1), Layout::xstep(i, -1), sy, -sy, sz, -sz};
     for (int r = 0; r < 6; ++r) {
         atomAdd(&Sb[o + nbs[r]], br);
         atomAdd(&Sb[o + nbs[r] + E4], be);
@@ -409,8 +409,11 @@ __global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const dou
     if (c[ax1] > hi[ax1] || c[ax2] > hi[ax2]) return;
     const size_t F = L.fstride;
     const size_t o = L.off(c[0], c[1], c[2]);
-    const long st = f.axis == 0 ? 1 : (f.axis == 1 ? (long)L.pitch : (long)L.plane);
-    const long s1 = f.side == 0 ? st : -st;
+    // offsets of the nodes one and two in from the face (x-steps depend on the column: padding slots)
+    const int dn = f.side == 0 ? 1 : -1;
+    const long st = f.axis == 1 ? (long)L.pitch : (long)L.plane;
+    const long s1 = f.axis == 0 ? Layout::xstep(c[0], dn) : dn * st;
+    const long s2 = f.axis == 0 ? Layout::xstep(c[0], 2 * dn) : 2 * dn * st;
     double ob[5];
     for (int r = 0; r < 5; ++r) {
         ob[r] = Sb[o + r * F];
@@ -424,7 +427,7 @@ __global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const dou
         for (int r = 0; r < 5; ++r) atomAdd(&Sb[src + r * F], ob[r]);
     } else if (f.type == NS_BC_SYMMETRY) {
         // conserved mirror of the node two in: the normal momentum changes sign, all else is copied
-        const size_t a2 = o + 2 * s1;
+        const size_t a2 = o + s2;
         for (int r = 0; r < 5; ++r) atomAdd(&Sb[a2 + r * F], r == 1 + f.axis ? -ob[r] : ob[r]);
     } else if (f.type == NS_BC_OUTLET) {
         // r, m extrapolated (2 x1 - x2); rhoE = |m|^2 / (2 (1 + r)) (p' = 0)
@@ -433,7 +436,7 @@ __global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const dou
         const double K = (mu_ * mu_ + mv * mv + mw * mw) * ir * ir;
         const double gr = ob[0] - 0.5 * K * ob[4];
         const double gm[3] = {ob[1] + ob[4] * mu_ * ir, ob[2] + ob[4] * mv * ir, ob[3] + ob[4] * mw * ir};
-        const size_t x1 = o + s1, x2 = o + 2 * s1;
+        const size_t x1 = o + s1, x2 = o + s2;
         atomAdd(&Sb[x1], 2.0 * gr);
         atomAdd(&Sb[x2], -gr);
         for (int r = 0; r < 3; ++r) {
@@ -457,20 +460,21 @@ __global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const dou
         if (qb && P.inflow) qb[qi] += bvel * ramp;  // one face node per profile entry: no race
         else if (qb) qb[(size_t)c[ax2] * (ax1 == 0 ? L.NI : L.NJ) + c[ax1]] += bvel * ramp;
         // acc = (sum_nb extrap(nb) + n extrap(o)) / (2 n), extrap(x) = 2 rho(x + s1) - rho(x + 2 s1)
-        const long stA[3] = {1, (long)L.pitch, (long)L.plane};
         unsigned nbn = 0;
         long nbo[4];
         for (int ax : {ax1, ax2}) {
-            if (c[ax] < hi[ax]) nbo[nbn++] = stA[ax];
-            if (c[ax] > lo[ax]) nbo[nbn++] = -stA[ax];
+            const long up = ax == 0 ? Layout::xstep(c[0], 1) : (ax == 1 ? (long)L.pitch : (long)L.plane);
+            const long down = ax == 0 ? Layout::xstep(c[0], -1) : -up;
+            if (c[ax] < hi[ax]) nbo[nbn++] = up;
+            if (c[ax] > lo[ax]) nbo[nbn++] = down;
         }
         const double wnb = bacc / (2.0 * nbn);
         const double wo = bacc * (double)nbn / (2.0 * nbn);
         atomAdd(&Sb[o + s1], 2.0 * wo);
-        atomAdd(&Sb[o + 2 * s1], -wo);
+        atomAdd(&Sb[o + s2], -wo);
         for (unsigned r = 0; r < nbn; ++r) {
             atomAdd(&Sb[o + nbo[r] + s1], 2.0 * wnb);
-            atomAdd(&Sb[o + nbo[r] + 2 * s1], -wnb);
+            atomAdd(&Sb[o + nbo[r] + s2], -wnb);
         }
     }
 }

@@ -321,17 +321,21 @@ __device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, c
     Fields<T> F = fields(L, S);
     Eos<T> e = Eos<T>::make(P);
     size_t o = L.off(c[0], c[1], c[2]);
-    long st = f.axis == 0 ? 1 : (f.axis == 1 ? (long)L.pitch : (long)L.plane);
-    long s1 = f.side == 0 ? st : -st;  // towards the interior
+    // offsets of the nodes one and two in from the face (x-steps depend on the column: padding slots)
+    const int dn = f.side == 0 ? 1 : -1;  // towards the interior
+    const long st = f.axis == 1 ? (long)L.pitch : (long)L.plane;
+    const long s1 = f.axis == 0 ? Layout::xstep(c[0], dn) : dn * st;
+    const long s2 = f.axis == 0 ? Layout::xstep(c[0], 2 * dn) : 2 * dn * st;
     if (f.type == NS_BC_INLET) {
-        auto extrap = [&](size_t oo) { return (T)2 * F.f[0][oo + s1] - F.f[0][oo + 2 * s1]; };
+        auto extrap = [&](size_t oo) { return (T)2 * F.f[0][oo + s1] - F.f[0][oo + s2]; };
         // filterInletDensity, src/Boundary.cpp:142-245: neighbours inside the owned box, order +a1,-a1,+a2,-a2
         T acc = 0;
         unsigned nb = 0;
-        long stA[3] = {1, (long)L.pitch, (long)L.plane};
         for (int ax : {ax1, ax2}) {
-            if (c[ax] < hi[ax]) { acc += extrap(o + stA[ax]); ++nb; }
-            if (c[ax] > lo[ax]) { acc += extrap(o - stA[ax]); ++nb; }
+            const long up = ax == 0 ? Layout::xstep(c[0], 1) : (ax == 1 ? (long)L.pitch : (long)L.plane);
+            const long down = ax == 0 ? Layout::xstep(c[0], -1) : -up;
+            if (c[ax] < hi[ax]) { acc += extrap(o + up); ++nb; }
+            if (c[ax] > lo[ax]) { acc += extrap(o + down); ++nb; }
         }
         acc += (T)nb * extrap(o);
         acc /= (T)(2 * nb);
@@ -348,10 +352,10 @@ __device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, c
         deriveCons(e, uu, vv, ww, p, Tt, rho, ru, rv, rw, rE);
         F.f[0][o] = rho; F.f[1][o] = ru; F.f[2][o] = rv; F.f[3][o] = rw; F.f[4][o] = rE;
     } else if (f.type == NS_BC_OUTLET) {
-        T r = (T)2 * F.f[0][o + s1] - F.f[0][o + 2 * s1];
-        T ru = (T)2 * F.f[1][o + s1] - F.f[1][o + 2 * s1];
-        T rv = (T)2 * F.f[2][o + s1] - F.f[2][o + 2 * s1];
-        T rw = (T)2 * F.f[3][o + s1] - F.f[3][o + 2 * s1];
+        T r = (T)2 * F.f[0][o + s1] - F.f[0][o + s2];
+        T ru = (T)2 * F.f[1][o + s1] - F.f[1][o + s2];
+        T rv = (T)2 * F.f[2][o + s1] - F.f[2][o + s2];
+        T rw = (T)2 * F.f[3][o + s1] - F.f[3][o + s2];
         T p = 0;
         T u = ru / ((T)1 + r), v = rv / ((T)1 + r), w = rw / ((T)1 + r);
         T rE = p / e.gm1 + ((T)1 + r) / (T)2 * (u * u + v * v + w * w);
@@ -363,7 +367,7 @@ __device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, c
         size_t src = L.off(cc[0], cc[1], cc[2]);
         for (int q = 0; q < 5; ++q) F.f[q][o] = F.f[q][src];
     } else {  // symmetry: mirror the primitives of the node two in, negate the normal velocity
-        size_t a2 = o + 2 * s1;
+        size_t a2 = o + s2;
         T u, v, w, p, Tt, mu;
         derivePrims(e, F.f[0][a2], F.f[1][a2], F.f[2][a2], F.f[3][a2], F.f[4][a2], u, v, w, p, Tt, mu);
         u = f.axis == 0 ? -u : u;  // selects, see the inlet above
@@ -408,10 +412,10 @@ __device__ __forceinline__ void filterComputeNode(const Layout& L, const Window&
     size_t o = L.off(i, j, k);
     const T* r = S;
     const T* E = S + 4 * L.fstride;
-    const long sx = 1, sy = L.pitch, sz = (long)L.plane;
+    const long xp = Layout::xstep(i, 1), xm = Layout::xstep(i, -1), sy = L.pitch, sz = (long)L.plane;
     // order of the reference sum: P*6, i+1, i-1, j+1, j-1, k+1, k-1
-    T fr = ((T)6 * r[o] + r[o + sx] + r[o - sx] + r[o + sy] + r[o - sy] + r[o + sz] + r[o - sz]) / (T)12;
-    T fe = ((T)6 * E[o] + E[o + sx] + E[o - sx] + E[o + sy] + E[o - sy] + E[o + sz] + E[o - sz]) / (T)12;
+    T fr = ((T)6 * r[o] + r[o + xp] + r[o + xm] + r[o + sy] + r[o - sy] + r[o + sz] + r[o - sz]) / (T)12;
+    T fe = ((T)6 * E[o] + E[o + xp] + E[o + xm] + E[o + sy] + E[o - sy] + E[o + sz] + E[o - sz]) / (T)12;
     tmp[2 * q] = fr;
     tmp[2 * q + 1] = fe;
     size_t w = W.idx(i, j, k);

@@ -1,10 +1,15 @@
 /*
  * ns_internal.h — shared host/device declarations of the MI355X Navier–Stokes step.
  *
- * Device field layout: structure of arrays, x fastest. Every x-row is padded to a multiple of
- * 128 bytes (pitch), every field starts on a 256-byte boundary; offset(i,j,k) = (k*NJ + j)*pitch + i.
- * The five conserved fields of one state (rho', rho*u, rho*v, rho*w, rho*E') are consecutive fields of
- * one allocation.
+ * Device field layout: structure of arrays, x fastest, x-rows in segments. The nodes of a row are cut into
+ * segments of 62 consecutive nodes; segment s occupies the 64 slots 64 s .. 64 s + 63 of the row and node
+ * i = 62 s + r sits in slot 64 s + r + 1, so slots 0 and 63 of every segment hold no node (padding). A
+ * segment is then 256 B (FP32) or 512 B (FP64) and starts on such a boundary: the fused stage kernel,
+ * whose x-tiles are the segments, writes whole aligned cache lines (a line written in part costs a
+ * read-modify-write in the memory system). Every x-row is padded to a multiple of 128 bytes (pitch),
+ * every field starts on a 256-byte boundary; offset(i,j,k) = (k*NJ + j)*pitch + col(i). The five
+ * conserved fields of one state (rho', rho*u, rho*v, rho*w, rho*E') are consecutive fields of one
+ * allocation.
  */
 #pragma once
 
@@ -34,12 +39,18 @@ enum : uint8_t {
 };
 
 struct Layout {
+    static constexpr int kSegNodes = 62;  // nodes per x-segment
+    static constexpr int kSegSlots = 64;  // slots per x-segment (first and last are padding)
     int NI, NJ, NK;
-    int pitch;        // elements per padded x-row
+    int pitch;        // slots per padded x-row
     size_t plane;     // NJ * pitch
     size_t fstride;   // elements between consecutive fields (>= NK * plane, 256-B aligned)
     NS_HD size_t nodes() const { return (size_t)NI * NJ * NK; }
-    NS_HD size_t off(int i, int j, int k) const { return ((size_t)k * NJ + j) * pitch + i; }
+    // slot of node i in its row (i >= -1; node -1 maps to the padding slot 0)
+    NS_HD static int col(int i) { return i + 2 * (i / kSegNodes) + 1; }
+    NS_HD size_t off(int i, int j, int k) const { return ((size_t)k * NJ + j) * pitch + col(i); }
+    // off(i + d, j, k) - off(i, j, k): the x-step of d nodes from node i (crosses padding slots)
+    NS_HD static long xstep(int i, int d) { return (long)col(i + d) - col(i); }
 };
 
 Layout makeLayout(int NI, int NJ, int NK, size_t elemBytes);

@@ -36,7 +36,8 @@ Layout makeLayout(int NI, int NJ, int NK, size_t eb) {
     L.NJ = NJ;
     L.NK = NK;
     size_t rowElems = 128 / eb;
-    L.pitch = (int)(((size_t)NI + rowElems - 1) / rowElems * rowElems);
+    const size_t slots = (size_t)Layout::col(NI - 1) + 1;  // through the last node's slot
+    L.pitch = (int)((slots + rowElems - 1) / rowElems * rowElems);
     L.plane = (size_t)NJ * L.pitch;
     size_t fe = 256 / eb;
     L.fstride = ((size_t)NK * L.plane + fe - 1) / fe * fe;

@@ -290,11 +290,11 @@ struct Raw {
     bool filt;
 };
 
-// Byte offset of node (i, j, k): 32-bit in FP32 (a state buffer of five fields stays below 4 GiB), the
-// column part per thread and the plane part uniform.
+// Byte offset of slot s of row j in a plane: 32-bit (a field stays below 4 GiB), the column part per
+// thread and the plane part uniform.
 template <class T>
-__device__ __forceinline__ uint32_t colBytes(const Layout& L, int i, int j) {
-    return (uint32_t)(j * L.pitch + i) * (uint32_t)sizeof(T);
+__device__ __forceinline__ uint32_t slotBytes(const Layout& L, int s, int j) {
+    return (uint32_t)(j * L.pitch + s) * (uint32_t)sizeof(T);
 }
 
 template <class T, class M>
@@ -378,12 +378,15 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     static_assert(!KAHAN || sizeof(T) == 4, "the compensated combine is the FP32 form");
     static_assert(!BUF || sizeof(T) == 4, "buffer-resource access is the FP32 form");
     const Mem<T, BUF> m(a);
-    // clamped column of the stencil point (halo lanes and rows beyond the grid repeat the edge) and the
-    // own column's byte offset
+    // clamped column of the stencil point (halo lanes and rows beyond the grid repeat the edge); the x-tile
+    // is segment bx of the layout, so the halo lanes read the neighbouring segments
     const int ic = min(max(i, 0), L.NI - 1), jc = min(max(j, 0), L.NJ - 1);
-    const uint32_t colc = colBytes<T>(L, ic, jc);
-    const uint32_t colo = colBytes<T>(L, i, j);
+    const uint32_t colc = slotBytes<T>(L, Layout::col(ic), jc);
+    // the own slot: lane l writes slot l of segment bx (lanes 0 and 63 the padding slots)
+    const uint32_t colo = slotBytes<T>(L, Layout::kSegSlots * bx + lane, j);
     const uint32_t planeB = (uint32_t)(L.plane * sizeof(T));
+    // padding slots of this segment that exist in the row (slot 63 only if the next segment does)
+    const bool padLane = lane == 0 || (lane == 63 && i < L.NI);
 
     // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
     Pt<T> qm, q0, q1;
@@ -416,7 +419,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
         rnext = fetch(a, m, ic, jc, colc, k + 2);
         // combine inputs of the output node, fetched before the stencil work
         const bool doOut = outRow && outLane;
-        const uint32_t bo = doOut ? colo + (uint32_t)k * planeB : 0;
+        const uint32_t bo = outRow ? colo + (uint32_t)k * planeB : 0;
         const size_t o = bo / sizeof(T);
         bool interior = false, filterNode = false;
         T base[5], acc[5];
@@ -490,9 +493,20 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                 rhs[q] += phix - shr1(phix);
                 rhs[q] += phiz[q] - phizm[q];
             }
+            // Values written at this slot. Each field goes out in one store for the whole wave: output
+            // lanes with their results, the padding lanes with the node's current values (never read),
+            // so a wave whose nodes all write covers its segment's lines in full.
+            T cur[5] = {q0.rho, q0.ru, q0.rv, q0.rw, q0.E};
+            T vS[5], vA[5];
+            uint16_t vC[5];
+            bool wS = padLane, wA = padLane, wC = padLane;
+            for (int q = 0; q < 5; ++q) {
+                vS[q] = vA[q] = cur[q];
+                vC[q] = 0;
+            }
             if (doOut) {
-                T cur[5] = {q0.rho, q0.ru, q0.rv, q0.rw, q0.E};
                 if (interior) {
+                    wS = wA = wC = true;
                     if (STAGE == 1) {
                         const T h2 = (T)(dtd / 2), h6 = (T)(dtd / 6);
                         // the reference filters the density of U before the step but keeps the
@@ -500,14 +514,14 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                         const T c0 = cur[0];
                         cur[0] = rhoFiltered ? ldAt(a.fRho, bo) : c0;
                         for (int q = 0; q < 5; ++q) {
-                            m.stSout(q, bo, cur[q] + h2 * rhs[q]);
-                            m.stA(q, bo, KAHAN ? h6 * rhs[q] : cur[q] + h6 * rhs[q]);
+                            vS[q] = cur[q] + h2 * rhs[q];
+                            vA[q] = KAHAN ? h6 * rhs[q] : cur[q] + h6 * rhs[q];
                         }
                     } else if (STAGE == 2 || STAGE == 3) {
                         const T hs = (T)(STAGE == 2 ? dtd / 2 : dtd), h3 = (T)(dtd / 3);
                         for (int q = 0; q < 5; ++q) {
-                            m.stSout(q, bo, base[q] + hs * rhs[q]);
-                            m.stA(q, bo, acc[q] + h3 * rhs[q]);
+                            vS[q] = base[q] + hs * rhs[q];
+                            vA[q] = acc[q] + h3 * rhs[q];
                         }
                     } else {
                         const T h6 = (T)(dtd / 6);
@@ -518,9 +532,9 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                                 // U^{n+1} = U^n + (D + e^{n-1}), its rounding error kept for the next step
                                 T e;
                                 an[q] = twoSum(base[q], an[q] + bf16Decode(m.ldComp(q, bo)), e);
-                                m.stComp(q, bo, bf16Encode(e));
+                                vC[q] = bf16Encode(e);
                             }
-                            m.stA(q, bo, an[q]);
+                            vA[q] = an[q];
                         }
                         // end-of-step scans; filter nodes change once more in the boundary pass and are
                         // scanned there (src/Solver.cpp:91-126, src/Mesh.cpp:171-178, src/Mesh.h:48-61)
@@ -532,11 +546,21 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                                 }
                             Pt<T> pn;
                             prims(c, an[0], an[1], an[2], an[3], an[4], pn);
-                            T cs = sqrt(((T)1 + c.gamma * pn.p) / ((T)1 + an[0]));
+                            T cs, nu;
+                            if constexpr (sizeof(T) == 4) {
+                                // FP32 scans only bound the next step size: hardware reciprocal and
+                                // square root (1 ulp) instead of the correctly rounded sequences
+                                const T im = rcp((T)1 + an[0]);
+                                cs = sqrtT(((T)1 + c.gamma * pn.p) * im);
+                                nu = pn.mu * im * a.vm;
+                            } else {
+                                cs = sqrt(((T)1 + c.gamma * pn.p) / ((T)1 + an[0]));
+                                nu = pn.mu / (an[0] + (T)1) * a.vm;
+                            }
                             red[0] = fmax(red[0], cs + fabs(pn.u));
                             red[1] = fmax(red[1], cs + fabs(pn.v));
                             red[2] = fmax(red[2], cs + fabs(pn.w));
-                            red[3] = fmax(red[3], pn.mu / (an[0] + (T)1) * a.vm);
+                            red[3] = fmax(red[3], nu);
                             for (int q = 0; q < 5; ++q)
                                 if (!isfinite(an[q])) red[9] += 1;
                         }
@@ -544,16 +568,24 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                 } else {
                     if (STAGE == 1) {
                         // the reference's target array holds the unfiltered density of U after the
-                        // filter's buffer swap (src/Mesh.cpp:133), the other fields of the step before
+                        // filter's buffer swap (src/Mesh.cpp:133), the other fields of the step before;
+                        // A keeps its values there
                         if (rhoFiltered) acc[0] = cur[0];
-                        for (int q = 0; q < 5; ++q) m.stSout(q, bo, acc[q]);
+                        for (int q = 0; q < 5; ++q) vS[q] = acc[q];
+                        wS = true;
+                    } else if (STAGE == 4) {
+                        wA = true;  // the node's current values
+                    } else {
+                        wS = true;
                     }
-                    else if (STAGE == 4)
-                        for (int q = 0; q < 5; ++q) m.stA(q, bo, cur[q]);
-                    else
-                        for (int q = 0; q < 5; ++q) m.stSout(q, bo, cur[q]);
                 }
             }
+            if (STAGE != 4 && wS)
+                for (int q = 0; q < 5; ++q) m.stSout(q, bo, vS[q]);
+            if (wA)
+                for (int q = 0; q < 5; ++q) m.stA(q, bo, vA[q]);
+            if (STAGE == 4 && KAHAN && wC)
+                for (int q = 0; q < 5; ++q) m.stComp(q, bo, vC[q]);
         }
         // rotate the column
         umz = q0.u;

@@ -88,8 +88,9 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
     const T* Tt = prim + 4 * F;
     const T* mu = prim + 5 * F;
     const T* ka = prim + 6 * F;
-    const long X = 1, Y = L.pitch, Z = (long)L.plane;
-    const size_t e_ = o + X, w_ = o - X, n_ = o + Y, s_ = o - Y, u_ = o + Z, d_ = o - Z;
+    // x-steps from column i (a step can cross the padding slots between segments)
+    const long XP = Layout::xstep(i, 1), XM = Layout::xstep(i, -1), Y = L.pitch, Z = (long)L.plane;
+    const size_t e_ = o + XP, w_ = o + XM, n_ = o + Y, s_ = o - Y, u_ = o + Z, d_ = o - Z;
     T kq[5];
     // continuity (src/Solver.cpp:191-194)
     kq[0] = -(ru[e_] - ru[w_]) / ((T)2 * dx) + -(rv[n_] - rv[s_]) / ((T)2 * dy) + -(rw[u_] - rw[d_]) / ((T)2 * dz);
@@ -101,9 +102,9 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         T vXX = p23xx * ((mu[e_] + mu[o]) * (u[e_] - u[o]) - (mu[w_] + mu[o]) * (u[o] - u[w_]));
         T vYY = p12yy * ((mu[n_] + mu[o]) * (u[n_] - u[o]) - (mu[s_] + mu[o]) * (u[o] - u[s_]));
         T vZZ = p12zz * ((mu[u_] + mu[o]) * (u[u_] - u[o]) - (mu[d_] + mu[o]) * (u[o] - u[d_]));
-        T vXY = c4xy * (mu[n_] * (v[n_ + X] - v[n_ - X]) - mu[s_] * (v[s_ + X] - v[s_ - X]));
+        T vXY = c4xy * (mu[n_] * (v[n_ + XP] - v[n_ + XM]) - mu[s_] * (v[s_ + XP] - v[s_ + XM]));
         T vYX = c6xy * (mu[e_] * (v[e_ + Y] - v[e_ - Y]) - mu[w_] * (v[w_ + Y] - v[w_ - Y]));
-        T vXZ = c4xz * (mu[u_] * (w[u_ + X] - w[u_ - X]) - mu[d_] * (w[d_ + X] - w[d_ - X]));
+        T vXZ = c4xz * (mu[u_] * (w[u_ + XP] - w[u_ + XM]) - mu[d_] * (w[d_ + XP] - w[d_ + XM]));
         T vZX = c6xz * (mu[e_] * (w[e_ + Z] - w[e_ - Z]) - mu[w_] * (w[w_ + Z] - w[w_ - Z]));
         kq[1] = cX + cY + cZ + vXX + vYY + vZZ + vXY + vYX + vXZ + vZX;
     }
@@ -115,7 +116,7 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         T vYY = p23yy * ((mu[n_] + mu[o]) * (v[n_] - v[o]) - (mu[s_] + mu[o]) * (v[o] - v[s_]));
         T vZZ = p12zz * ((mu[u_] + mu[o]) * (v[u_] - v[o]) - (mu[d_] + mu[o]) * (v[o] - v[d_]));
         T vYX = c4xy * (mu[e_] * (u[e_ + Y] - u[e_ - Y]) - mu[w_] * (u[w_ + Y] - u[w_ - Y]));
-        T vXY = c6xy * (mu[n_] * (u[n_ + X] - u[n_ - X]) - mu[s_] * (u[s_ + X] - u[s_ - X]));
+        T vXY = c6xy * (mu[n_] * (u[n_ + XP] - u[n_ + XM]) - mu[s_] * (u[s_ + XP] - u[s_ + XM]));
         T vYZ = c4yz * (mu[u_] * (w[u_ + Y] - w[u_ - Y]) - mu[d_] * (w[d_ + Y] - w[d_ - Y]));
         T vZY = c6yz * (mu[n_] * (w[n_ + Z] - w[n_ - Z]) - mu[s_] * (w[s_ + Z] - w[s_ - Z]));
         kq[2] = cX + cY + cZ + vXX + vYY + vZZ + vYX + vXY + vYZ + vZY;
@@ -128,7 +129,7 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         T vYY = p12yy * ((mu[n_] + mu[o]) * (w[n_] - w[o]) - (mu[s_] + mu[o]) * (w[o] - w[s_]));
         T vZZ = p23zz * ((mu[u_] + mu[o]) * (w[u_] - w[o]) - (mu[d_] + mu[o]) * (w[o] - w[d_]));
         T vZX = c4xz * (mu[e_] * (u[e_ + Z] - u[e_ - Z]) - mu[w_] * (u[w_ + Z] - u[w_ - Z]));
-        T vXZ = c6xz * (mu[u_] * (u[u_ + X] - u[u_ - X]) - mu[d_] * (u[d_ + X] - u[d_ - X]));
+        T vXZ = c6xz * (mu[u_] * (u[u_ + XP] - u[u_ + XM]) - mu[d_] * (u[d_ + XP] - u[d_ + XM]));
         T vZY = c4yz * (mu[n_] * (v[n_ + Z] - v[n_ - Z]) - mu[s_] * (v[s_ + Z] - v[s_ - Z]));
         T vYZ = c6yz * (mu[u_] * (v[u_ + Y] - v[u_ - Y]) - mu[d_] * (v[d_ + Y] - v[d_ - Y]));
         kq[3] = cX + cY + cZ + vXX + vYY + vZZ + vZX + vXZ + vZY + vYZ;
@@ -153,16 +154,16 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         T vy = p12yy * ((mN * uN + mP * uP) * (uN - uP) - (mS * uS + mP * uP) * (uP - uS))
              + p23yy * ((mN * vN + mP * vP) * (vN - vP) - (mS * vS + mP * vP) * (vP - vS))
              + p12yy * ((mN * wN + mP * wP) * (wN - wP) - (mS * wS + mP * wP) * (wP - wS))
-             + c6xy * (mN * vN * (u[n_ + X] - u[n_ - X]) - mS * vS * (u[s_ + X] - u[s_ - X]))
+             + c6xy * (mN * vN * (u[n_ + XP] - u[n_ + XM]) - mS * vS * (u[s_ + XP] - u[s_ + XM]))
              + c6yz * (mN * vN * (w[n_ + Z] - w[n_ - Z]) - mS * vS * (w[s_ + Z] - w[s_ - Z]))
-             + c4xy * (mN * uN * (v[n_ + X] - v[n_ - X]) - mS * uS * (v[s_ + X] - v[s_ - X]))
+             + c4xy * (mN * uN * (v[n_ + XP] - v[n_ + XM]) - mS * uS * (v[s_ + XP] - v[s_ + XM]))
              + c4yz * (mN * wN * (v[n_ + Z] - v[n_ - Z]) - mS * wS * (v[s_ + Z] - v[s_ - Z]));
         T vz = p12zz * ((mU * uU + mP * uP) * (uU - uP) - (mD * uD + mP * uP) * (uP - uD))
              + p12zz * ((mU * vU + mP * vP) * (vU - vP) - (mD * vD + mP * vP) * (vP - vD))
              + p23zz * ((mU * wU + mP * wP) * (wU - wP) - (mD * wD + mP * wP) * (wP - wD))
-             + c6xz * (mU * wU * (u[u_ + X] - u[u_ - X]) - mD * wD * (u[d_ + X] - u[d_ - X]))
+             + c6xz * (mU * wU * (u[u_ + XP] - u[u_ + XM]) - mD * wD * (u[d_ + XP] - u[d_ + XM]))
              + c6yz * (mU * wU * (v[u_ + Y] - v[u_ - Y]) - mD * wD * (v[d_ + Y] - v[d_ - Y]))
-             + c4xz * (mU * uU * (w[u_ + X] - w[u_ - X]) - mD * uD * (w[d_ + X] - w[d_ - X]))
+             + c4xz * (mU * uU * (w[u_ + XP] - w[u_ + XM]) - mD * uD * (w[d_ + XP] - w[d_ + XM]))
              + c4yz * (mU * vU * (w[u_ + Y] - w[u_ - Y]) - mD * vD * (w[d_ + Y] - w[d_ - Y]));
         T heat = p12xx * ((ka[e_] + kP) * (Tt[e_] - TP) - (ka[w_] + kP) * (TP - Tt[w_]))
                + p12yy * ((ka[n_] + kP) * (Tt[n_] - TP) - (ka[s_] + kP) * (TP - Tt[s_]))
@@ -269,9 +270,9 @@ __global__ void k_rho_filter(Layout L, const T* U, T* out, const uint8_t* cls, c
     if (i >= L.NI) return;
     size_t o = L.off(i, j, k);
     if ((cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) return;
-    const long X = 1, Y = L.pitch, Z = (long)L.plane;
+    const long XP = Layout::xstep(i, 1), XM = Layout::xstep(i, -1), Y = L.pitch, Z = (long)L.plane;
     // (6 rho_P + sum of the 6 face neighbours) / 12, in the reference's summation order
-    out[o] = ((T)6 * U[o] + U[o + X] + U[o - X] + U[o + Y] + U[o - Y] + U[o + Z] + U[o - Z]) / (T)12;
+    out[o] = ((T)6 * U[o] + U[o + XP] + U[o + XM] + U[o + Y] + U[o - Y] + U[o + Z] + U[o - Z]) / (T)12;
 }
 
 template <class T>
