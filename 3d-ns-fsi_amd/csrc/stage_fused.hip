@@ -129,6 +129,10 @@ template <class T>
 __device__ __forceinline__ void stAt(T* base, uint32_t boff, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
+template <class T>
+__device__ __forceinline__ void stNT(T* base, uint32_t boff, T v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff));
+}
 
 // Field accessors of one stage launch. BUF (FP32 states of at most 4 GiB): buffer instructions, one
 // resource per 5-field state buffer, the node's byte offset in a VGPR and the field's in an SGPR, so no
@@ -140,6 +144,9 @@ struct Mem;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
 }
+
+// cache-policy operand of the buffer intrinsics: non-temporal (streaming)
+constexpr int kNT = 2;
 
 template <>
 struct Mem<float, true> {
@@ -157,8 +164,10 @@ struct Mem<float, true> {
     __device__ float ld(__amdgpu_buffer_rsrc_t r, int q, uint32_t bo) const {
         return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)bo, (int)(q * fsb), 0));
     }
+    // stores are streaming (non-temporal): every output wave writes whole lines that this launch never
+    // reads again
     __device__ void st(__amdgpu_buffer_rsrc_t r, int q, uint32_t bo, float v) const {
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)bo, (int)(q * fsb), 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)bo, (int)(q * fsb), kNT);
     }
     __device__ float ldSin(int q, uint32_t bo) const { return ld(sin, q, bo); }
     __device__ float ldU(int q, uint32_t bo) const { return ld(u, q, bo); }
@@ -170,7 +179,7 @@ struct Mem<float, true> {
         return __builtin_amdgcn_raw_buffer_load_b16(comp, (int)(bo >> 1), (int)(q * (fsb >> 1)), 0);
     }
     __device__ void stComp(int q, uint32_t bo, uint16_t v) const {
-        __builtin_amdgcn_raw_buffer_store_b16(v, comp, (int)(bo >> 1), (int)(q * (fsb >> 1)), 0);
+        __builtin_amdgcn_raw_buffer_store_b16(v, comp, (int)(bo >> 1), (int)(q * (fsb >> 1)), kNT);
     }
 };
 
@@ -181,14 +190,14 @@ struct Mem<T, false> {
     __device__ T ldSin(int q, uint32_t bo) const { return ldAt(x.Sin[q], bo); }
     __device__ T ldU(int q, uint32_t bo) const { return ldAt(x.U[q], bo); }
     __device__ T ldA(int q, uint32_t bo) const { return ldAt((const T*)x.A[q], bo); }
-    __device__ void stA(int q, uint32_t bo, T v) const { stAt(x.A[q], bo, v); }
-    __device__ void stSout(int q, uint32_t bo, T v) const { stAt(x.Sout[q], bo, v); }
+    __device__ void stA(int q, uint32_t bo, T v) const { stNT(x.A[q], bo, v); }
+    __device__ void stSout(int q, uint32_t bo, T v) const { stNT(x.Sout[q], bo, v); }
     __device__ uint16_t ldComp(int q, uint32_t bo) const {
         if constexpr (sizeof(T) == 4) return ldAt((const uint16_t*)x.comp[q], bo >> 1);
         else return 0;
     }
     __device__ void stComp(int q, uint32_t bo, uint16_t v) const {
-        if constexpr (sizeof(T) == 4) stAt(x.comp[q], bo >> 1, v);
+        if constexpr (sizeof(T) == 4) stNT(x.comp[q], bo >> 1, v);
     }
 };
 
