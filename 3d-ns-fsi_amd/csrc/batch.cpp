@@ -234,7 +234,8 @@ double Batch::autotune(int reps) {
     if (!initialized) throw StateError("autotune before init");
     int nh = 0;
     const int* tys = fusedTileHeights(&nh);
-    const int kcs[4] = {16, 32, 64, 128};
+    int nk = 0;
+    const int* kcs = fusedChunkPlanes(&nk);
     hipEvent_t e0, e1;
     hipCheck(hipEventCreate(&e0), "event");
     hipCheck(hipEventCreate(&e1), "event");
@@ -243,7 +244,7 @@ double Batch::autotune(int reps) {
     double bestMs = 1e30;
     tuneTimes.clear();
     for (int a = 0; a < nh; ++a)
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < nk; ++b) {
             tile.ty = tys[a];
             tile.kc = kcs[b];
             buildStageTables();

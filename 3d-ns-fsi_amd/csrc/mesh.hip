@@ -484,63 +484,54 @@ __device__ __forceinline__ double luSolveDot(const double* LU, const int8_t* rp,
 
 // Ghost update over the compacted ghost list (src/Boundary.cpp:432-456): gather the 8 image-point box
 // values, interpolate (tensor-product lerp when all 8 are fluid, else the stored FP64 Vandermonde
-// factors), mirror into the ghost. Interpolation runs in FP64 whatever the field precision.
+// factors), mirror into the ghost. Interpolation runs in FP64 whatever the field precision. One thread
+// per (ghost, field): the five fields of a ghost are independent (a ghost corner that is itself a ghost
+// enters as 0 and is never read), so they run in parallel.
 template <class T>
-__device__ __forceinline__ void ghostApplyNode(const Layout& L, const Params& P, const GhostTables& G, int64_t g, T* S) {
+__device__ __forceinline__ void ghostApplyField(const Layout& L, const Params& P, const GhostTables& G, int64_t g, int q,
+                                                T* S) {
     const uint8_t mask = G.mask[g];
-    double val[5][8];
+    T* Sq = S + (size_t)q * L.fstride;
+    double f[8];
     for (int c = 0; c < 8; ++c) {
-        size_t o = (size_t)G.corner[8 * g + c];
-        bool gh = (mask >> c) & 1;
-        for (int q = 0; q < 5; ++q) val[q][c] = gh ? 0.0 : (double)S[o + (size_t)q * L.fstride];
+        const size_t o = (size_t)G.corner[8 * g + c];
+        f[c] = (mask >> c) & 1 ? 0.0 : (double)Sq[o];
     }
-    double ipv[5];
-    if (mask == 0) {
-        double rx = G.rel[3 * g], ry = G.rel[3 * g + 1], rz = G.rel[3 * g + 2];
-        for (int q = 0; q < 5; ++q) {  // simplifiedInterpolation, src/Boundary.cpp:732-757
-            const double* f = val[q];
-            double a01 = f[0] + (f[1] - f[0]) * rz / P.hz;
-            double a23 = f[2] + (f[3] - f[2]) * rz / P.hz;
-            double a45 = f[4] + (f[5] - f[4]) * rz / P.hz;
-            double a67 = f[6] + (f[7] - f[6]) * rz / P.hz;
-            double a0123 = a01 + (a23 - a01) * ry / P.hy;
-            double a4567 = a45 + (a67 - a45) * ry / P.hy;
-            ipv[q] = a0123 + (a4567 - a0123) * rx / P.hx;
-        }
+    double ipv;
+    if (mask == 0) {  // simplifiedInterpolation, src/Boundary.cpp:732-757
+        const double rx = G.rel[3 * g], ry = G.rel[3 * g + 1], rz = G.rel[3 * g + 2];
+        const double a01 = f[0] + (f[1] - f[0]) * rz / P.hz;
+        const double a23 = f[2] + (f[3] - f[2]) * rz / P.hz;
+        const double a45 = f[4] + (f[5] - f[4]) * rz / P.hz;
+        const double a67 = f[6] + (f[7] - f[6]) * rz / P.hz;
+        const double a0123 = a01 + (a23 - a01) * ry / P.hy;
+        const double a4567 = a45 + (a67 - a45) * ry / P.hy;
+        ipv = a0123 + (a4567 - a0123) * rx / P.hx;
     } else {
-        double x = G.ip[3 * g], y = G.ip[3 * g + 1], z = G.ip[3 * g + 2];
-        const double* D = G.luD + 64 * g;
-        const double* N = G.luN + 64 * g;
-        const int8_t* pm = G.perm + 32 * g;
-        ipv[0] = luSolveDot(N, pm + 16, pm + 24, val[0], x, y, z);
-        ipv[1] = luSolveDot(D, pm, pm + 8, val[1], x, y, z);
-        ipv[2] = luSolveDot(D, pm, pm + 8, val[2], x, y, z);
-        ipv[3] = luSolveDot(D, pm, pm + 8, val[3], x, y, z);
-        ipv[4] = luSolveDot(N, pm + 16, pm + 24, val[4], x, y, z);
+        // density and energy: Neumann factors; momentum: Dirichlet
+        const bool neu = q == 0 || q == 4;
+        const int8_t* pm = G.perm + 32 * g + (neu ? 16 : 0);
+        ipv = luSolveDot((neu ? G.luN : G.luD) + 64 * g, pm, pm + 8, f, G.ip[3 * g], G.ip[3 * g + 1], G.ip[3 * g + 2]);
     }
-    size_t o = (size_t)G.node[g];
-    // getGhostNodeBCVariables, src/Boundary.cpp:777-785
-    S[o] = (T)ipv[0];
-    S[o + L.fstride] = (T)(-ipv[1]);
-    S[o + 2 * L.fstride] = (T)(-ipv[2]);
-    S[o + 3 * L.fstride] = (T)(-ipv[3]);
-    S[o + 4 * L.fstride] = (T)ipv[4];
+    // getGhostNodeBCVariables, src/Boundary.cpp:777-785: the momentum changes sign
+    Sq[(size_t)G.node[g]] = (T)(q >= 1 && q <= 3 ? -ipv : ipv);
 }
 
 template <class T>
 __global__ void k_ghost_apply(Layout L, Params P, GhostTables G, int64_t n, T* S) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < n) ghostApplyNode(L, P, G, g, S);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 5 * n) ghostApplyField(L, P, G, t / 5, (int)(t % 5), S);
 }
 
 template <class T>
 __global__ void k_ghost_apply_batched(const CaseView<T>* __restrict__ cv, const int64_t* __restrict__ start, int ncases,
                                       int role, int parity) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = t / 5;
     if (q >= start[ncases]) return;
     const int c = findCase(start, ncases, q);
     const CaseView<T>& v = cv[c];
-    ghostApplyNode(v.L, v.P, v.G, q - start[c], roleBuffer(v, role, parity));
+    ghostApplyField(v.L, v.P, v.G, q - start[c], (int)(t % 5), roleBuffer(v, role, parity));
 }
 
 inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -623,7 +614,7 @@ void launchFilterWrite(hipStream_t s, const Layout& L, const int64_t* fn, int64_
 template <class T>
 void launchGhostApply(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, T* S) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_ghost_apply<T>, dim3(blocks(n, 64)), dim3(64), 0, s, L, P, G, n, S);
+    hipLaunchKernelGGL(k_ghost_apply<T>, dim3(blocks(5 * n, 64)), dim3(64), 0, s, L, P, G, n, S);
 }
 
 template <class T>
@@ -654,7 +645,8 @@ template <class T>
 void launchGhostApplyBatched(hipStream_t s, const CaseView<T>* cv, const int64_t* start, int64_t total, int ncases,
                              int role, int parity) {
     if (total == 0) return;
-    hipLaunchKernelGGL(k_ghost_apply_batched<T>, dim3(blocks(total, 64)), dim3(64), 0, s, cv, start, ncases, role, parity);
+    hipLaunchKernelGGL(k_ghost_apply_batched<T>, dim3(blocks(5 * total, 64)), dim3(64), 0, s, cv, start, ncases, role,
+                       parity);
 }
 
 #define NS_INST(T)                                                                                                           \

@@ -548,7 +548,8 @@ double Case::autotune(int reps) {
     // A's interior by every stage 1, so the state of the run is unchanged
     int n = 0;
     const int* tys = fusedTileHeights(&n);
-    const int kcs[4] = {16, 32, 64, 128};
+    int nk = 0;
+    const int* kcs = fusedChunkPlanes(&nk);
     hipEvent_t e0, e1;
     hipCheck(hipEventCreate(&e0), "event");
     hipCheck(hipEventCreate(&e1), "event");
@@ -556,7 +557,7 @@ double Case::autotune(int reps) {
     FusedTile bestTile = tile;
     tuneTimes.clear();
     for (int a = 0; a < n; ++a)
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < nk; ++b) {
             FusedTile tl;
             tl.ty = tys[a];
             tl.kc = kcs[b];

@@ -74,6 +74,7 @@ int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P,
                      const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
                      const int32_t* filterFlag, uint16_t* comp);
 const int* fusedTileHeights(int* n);
+const int* fusedChunkPlanes(int* n);  // z-chunk lengths the autotuners try
 // Batched FP32 fused stage: fusedArgsBytes() bytes of arguments per case (buildFusedArgs fills them and
 // returns the case's tile count); launchStageBatched runs stage `stage` of every case in one launch over
 // a device table of those blocks, start = device prefix sums of the tile counts (ncases + 1 entries).

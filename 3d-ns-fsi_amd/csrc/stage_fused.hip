@@ -956,6 +956,14 @@ const int* fusedTileHeights(int* n) {
     return v;
 }
 
+// z-chunk lengths tried by the autotuners: not only powers of two, since the number of blocks per launch
+// against the blocks the chip holds at once sets how idle the last round of blocks leaves it
+const int* fusedChunkPlanes(int* n) {
+    static const int v[10] = {12, 16, 20, 24, 32, 40, 48, 64, 96, 128};
+    *n = 10;
+    return v;
+}
+
 template int launchStageFused<double>(hipStream_t, int, const Layout&, const Params&, const Window&, const double*,
                                       const double*, double*, double*, const uint8_t*, const double*, const double*,
                                       const double*, FusedTile, double*, int, const double*, const int32_t*,

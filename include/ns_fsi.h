@@ -214,7 +214,8 @@ ns_status ns_profile(ns_handle* h, int32_t nsteps, double ms[4], int32_t launche
  *   "stage_kernel" 1 (default) fused stage kernel, 0 the unfused pair (primitive pass + per-node RHS);
  *   "tile_ty"      output rows per block of the fused stage (4, 6, 8 or 14; default 8);
  *   "tile_kc"      z-planes per block of the fused stage (default 64);
- *   "autotune"     after ns_init: time every (tile_ty, tile_kc) pair on this case with `value` launches
+ *   "autotune"     after ns_init: time every (tile_ty, tile_kc) pair (tile_kc from 12, 16, 20, 24, 32,
+ *                  40, 48, 64, 96, 128) on this case with `value` launches
  *                  each (the state is not changed) and keep the fastest;
  *   "fused_scans"  1 (default): the end-of-step scans (norms, dt maxima, finite check) run inside the
  *                  last fused stage kernel plus a small boundary pass; 0: one separate pass over the grid
