@@ -19,8 +19,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "3d-ns-fsi_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per stage launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy), per
-# workload configuration; a run whose configuration differs reports traffic as null.
+# HBM bytes per stage launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy) for one
+# workload form (workload, precision, combine) at one tile; another form reports traffic as null.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_fused_fp32_sphere512_traffic.json")
 VALUES_PER_NODE_STEP = 80  # 3-register RK4: stage 1 reads 5 writes 10, stages 2-3 read 15 write 10, stage 4 reads 10 writes 5
 
@@ -49,17 +49,17 @@ def cpu_baseline(workload, precision_unused=None):
     }
 
 
-def measured_traffic(workload, precision, kahan, tile, kernel):
+def measured_traffic(workload, precision, kahan, kernel):
+    """(bytes per stage launch, tile it was measured at) for this workload form, or (None, None)."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, None
     if (kernel == "fused" and t.get("workload") == workload and t.get("precision") == precision
-            and bool(t.get("compensated_combine")) == bool(kahan) and tile is not None
-            and list(t.get("tile", [])) == list(tile)):
-        return t.get("traffic_bytes_per_stage_launch")
-    return None
+            and bool(t.get("compensated_combine")) == bool(kahan)):
+        return t.get("traffic_bytes_per_stage_launch"), list(t.get("tile", []))
+    return None, None
 
 
 def copy_calibration(nbytes):
@@ -215,7 +215,7 @@ def main():
     achieved = alg_bytes / (ms[0] * 1e-3) / 1e9 if ms[0] > 0 else None
     g, f, _ = s.mesh.counts()
     kahan = bool(prec == "fp32" and cfg.kahan)
-    traffic = measured_traffic(a.workload, prec, kahan, tile, a.kernel)
+    traffic, traffic_tile = measured_traffic(a.workload, prec, kahan, a.kernel)
     traffic = round(traffic) if traffic else None
     line = {
         "metric": "M node-updates/s per RK4 step",
@@ -243,6 +243,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": traffic,
+            # the tile the traffic was measured at, when the autotuner chose another this run
+            "traffic_tile": traffic_tile if traffic and tile and list(tile) != traffic_tile else None,
             "kernel": "k_stage_fused (RK4 stage)" if a.kernel == "fused" else "k_rhs_plain (RK4 stage)",
             "kernel_ms": round(ms[0], 4),
             "step_breakdown_ms": {"stage_kernel": round(ms[0] * launches[0], 3),
