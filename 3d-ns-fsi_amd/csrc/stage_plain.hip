@@ -28,13 +28,36 @@ namespace nsfsi {
 
 namespace {
 
+// Both kernels run one block of 64 threads per x-segment of the layout, thread t on slot t: the padding
+// slots are written too (zeros, never read), so a segment's lines are written whole.
+struct SegSlot {
+    int i;        // node of the slot
+    size_t o;     // its offset
+    bool pad;     // padding slot (or past the last node)
+    bool write;   // whether the slot lies in the row (padding slots included)
+};
+__device__ __forceinline__ SegSlot segSlot(const Layout& L, int j, int k) {
+    SegSlot r;
+    const int t = threadIdx.x;
+    r.i = blockIdx.x * Layout::kSegNodes + t - 1;
+    r.o = ((size_t)k * L.NJ + j) * L.pitch + (size_t)blockIdx.x * Layout::kSegSlots + t;
+    r.pad = t == 0 || t == Layout::kSegSlots - 1 || r.i >= L.NI;
+    r.write = t == 0 || r.i < L.NI;  // slot 63 exists when the next segment does
+    return r;
+}
+
 template <class T>
 __global__ void k_derive(Layout L, Params P, const T* S, T* prim, const uint8_t* cls, Window W, const T* rhoPre,
                          const T* rhoEPre) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= L.NI) return;
-    size_t o = L.off(i, j, k);
+    const int j = blockIdx.y, k = blockIdx.z;
+    const SegSlot sl = segSlot(L, j, k);
+    if (sl.pad) {
+        if (sl.write)
+            for (int q = 0; q < 7; ++q) prim[sl.o + q * L.fstride] = (T)0;
+        return;
+    }
+    const int i = sl.i;
+    const size_t o = sl.o;
     Eos<T> e = Eos<T>::make(P);
     T rho = S[o], rE = S[o + 4 * L.fstride];
     if (cls[o] & CLS_FILTER) {
@@ -57,12 +80,22 @@ template <class T, int STAGE, bool KAHAN>
 __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, const T* U, T* Sout, T* A, const uint8_t* cls,
                             const double* dtdev, const T* fRho, const int32_t* fFlag, uint16_t* comp) {
     const bool rhoFiltered = STAGE == 1 && fFlag && *fFlag;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= L.NI) return;
-    const size_t o = L.off(i, j, k);
+    const int j = blockIdx.y, k = blockIdx.z;
+    const SegSlot sl = segSlot(L, j, k);
     const size_t F = L.fstride;
     T* out = STAGE == 4 ? A : Sout;
+    if (sl.pad) {
+        if (sl.write) {
+            for (int q = 0; q < 5; ++q) {
+                out[sl.o + q * F] = (T)0;
+                A[sl.o + q * F] = (T)0;
+                if (KAHAN && STAGE == 4) comp[sl.o + q * F] = 0;
+            }
+        }
+        return;
+    }
+    const int i = sl.i;
+    const size_t o = sl.o;
     if ((cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) {
         const T* src = STAGE == 1 ? A : Sin;
         for (int q = 0; q < 5; ++q) out[o + q * F] = src[o + q * F];
@@ -204,7 +237,7 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
 template <class T>
 void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T* prim, const uint8_t* cls, const Window& W,
                   const T* rhoPre, const T* rhoEPre) {
-    dim3 blk(64), grd((L.NI + 63) / 64, L.NJ, L.NK);
+    dim3 blk(Layout::kSegSlots), grd((L.NI + Layout::kSegNodes - 1) / Layout::kSegNodes, L.NJ, L.NK);
     hipLaunchKernelGGL(k_derive<T>, grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
 }
 
@@ -212,7 +245,7 @@ template <class T, bool K>
 void launchRhsPlainK(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                      T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag,
                      uint16_t* comp) {
-    dim3 blk(64), grd((L.NI + 63) / 64, L.NJ, L.NK);
+    dim3 blk(Layout::kSegSlots), grd((L.NI + Layout::kSegNodes - 1) / Layout::kSegNodes, L.NJ, L.NK);
     switch (stage) {
         case 1:
             hipLaunchKernelGGL((k_rhs_plain<T, 1, K>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho,
