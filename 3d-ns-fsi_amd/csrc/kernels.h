@@ -51,6 +51,11 @@ void launchGhostGeometry(hipStream_t s, const Layout& L, const uint8_t* cls, con
                          const int64_t* ghosts, int64_t n, const int32_t* gmap, GhostTables G, int32_t* error);
 
 // ---- boundary conditions on a state buffer (mesh.hip)
+// All six edge faces in application order, independent faces sharing one launch (same result as
+// launchEdgeFace face by face).
+template <class T>
+void launchEdgeFaces(hipStream_t s, const Layout& L, const Params& P, const EdgeFace faces[6], T* S, const double* tdev,
+                     const double* dtdev, double stageFrac);
 template <class T>
 void launchEdgeFace(hipStream_t s, const Layout& L, const Params& P, const EdgeFace& f, T* S, const double* tdev,
                     const double* dtdev, double stageFrac);

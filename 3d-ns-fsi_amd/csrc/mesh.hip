@@ -385,6 +385,26 @@ __global__ void k_edge_face(Layout L, Params P, EdgeFace f, T* S, const double* 
     edgeFaceNode(L, P, f, S, ts, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
+// Several edge faces in one launch (faces that neither write what another of them reads): face q takes
+// the blocks start[q] .. start[q+1]-1, laid out as its own (ceil(n1/64), n2) grid.
+struct EdgeFaceGroup {
+    EdgeFace f[6];
+    int start[7];
+    int nbx[6];
+    int n;
+};
+
+template <class T>
+__global__ void k_edge_faces(Layout L, Params P, EdgeFaceGroup g, T* S, const double* tdev, const double* dtdev,
+                             double frac) {
+    const int b = blockIdx.x;
+    int q = 0;
+    while (q + 1 < g.n && b >= g.start[q + 1]) ++q;
+    const int lb = b - g.start[q];
+    const double ts = *tdev + frac * *dtdev;
+    edgeFaceNode(L, P, g.f[q], S, ts, (lb % g.nbx[q]) * 64 + threadIdx.x, lb / g.nbx[q]);
+}
+
 template <class T>
 __global__ void k_edge_face_batched(const CaseView<T>* __restrict__ cv, const int64_t* __restrict__ start, int ncases,
                                     int face, int role, int parity, double frac) {
@@ -598,6 +618,72 @@ void launchEdgeFace(hipStream_t s, const Layout& L, const Params& P, const EdgeF
     hipLaunchKernelGGL(k_edge_face<T>, dim3(blocks(n1, 64), n2), dim3(64), 0, s, L, P, f, S, tdev, dtdev, frac);
 }
 
+namespace {
+
+// Inclusive index box.
+struct Box {
+    int lo[3], hi[3];
+    bool empty() const { return lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]; }
+    bool meets(const Box& o) const {
+        for (int a = 0; a < 3; ++a)
+            if (hi[a] < o.lo[a] || o.hi[a] < lo[a]) return false;
+        return true;
+    }
+};
+
+Box ownedBox(const EdgeFace& f) { return Box{{f.i0, f.j0, f.k0}, {f.i1, f.j1, f.k1}}; }
+
+// Nodes the face reads: the two planes in from it along its normal (inlet, outlet, symmetry; the inlet's
+// in-plane neighbours lie inside its own box), or the source plane of a periodic copy.
+Box readBox(const Layout& L, const EdgeFace& f) {
+    Box r = ownedBox(f);
+    const int a = f.axis, N = a == 0 ? L.NI : (a == 1 ? L.NJ : L.NK);
+    if (f.type == NS_BC_PERIODIC) {
+        r.lo[a] = r.hi[a] = f.side == 0 ? N - 2 : 1;
+    } else if (f.side == 0) {
+        r.lo[a] = r.lo[a] + 1;
+        r.hi[a] = r.lo[a] + 1;
+    } else {
+        r.hi[a] = r.hi[a] - 1;
+        r.lo[a] = r.hi[a] - 1;
+    }
+    return r;
+}
+
+bool independent(const Layout& L, const EdgeFace& x, const EdgeFace& y) {
+    return !readBox(L, x).meets(ownedBox(y)) && !readBox(L, y).meets(ownedBox(x)) && !ownedBox(x).meets(ownedBox(y));
+}
+
+}  // namespace
+
+template <class T>
+void launchEdgeFaces(hipStream_t s, const Layout& L, const Params& P, const EdgeFace faces[6], T* S, const double* tdev,
+                     const double* dtdev, double frac) {
+    // faces in application order, grouped greedily: a face joins the current launch when it is independent
+    // of every face already in it; the groups run in order, so the result equals face-by-face application
+    EdgeFaceGroup g{};
+    auto flush = [&]() {
+        if (g.n > 0 && g.start[g.n] > 0)
+            hipLaunchKernelGGL(k_edge_faces<T>, dim3(g.start[g.n]), dim3(64), 0, s, L, P, g, S, tdev, dtdev, frac);
+        g = EdgeFaceGroup{};
+    };
+    for (int q = 0; q < 6; ++q) {
+        const EdgeFace& f = faces[q];
+        if (ownedBox(f).empty()) continue;
+        bool ok = true;
+        for (int r = 0; r < g.n; ++r) ok = ok && independent(L, g.f[r], f);
+        if (!ok) flush();
+        const int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
+        const Box b = ownedBox(f);
+        const int n1 = b.hi[ax1] - b.lo[ax1] + 1, n2 = b.hi[ax2] - b.lo[ax2] + 1;
+        g.f[g.n] = f;
+        g.nbx[g.n] = (n1 + 63) / 64;
+        g.start[g.n + 1] = g.start[g.n] + g.nbx[g.n] * n2;
+        ++g.n;
+    }
+    flush();
+}
+
 template <class T>
 void launchFilterCompute(hipStream_t s, const Layout& L, const Window& W, const int64_t* fn, int64_t n, const T* S, T* tmp,
                          T* rhoPre, T* rhoEPre) {
@@ -655,6 +741,8 @@ void launchGhostApplyBatched(hipStream_t s, const CaseView<T>* cv, const int64_t
     template void launchFilterComputeBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);    \
     template void launchFilterWriteBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);      \
     template void launchGhostApplyBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);       \
+    template void launchEdgeFaces<T>(hipStream_t, const Layout&, const Params&, const EdgeFace[6], T*, const double*,         \
+                                     const double*, double);                                                                \
     template void launchEdgeFace<T>(hipStream_t, const Layout&, const Params&, const EdgeFace&, T*, const double*,            \
                                     const double*, double);                                                                   \
     template void launchFilterCompute<T>(hipStream_t, const Layout&, const Window&, const int64_t*, int64_t, const T*, T*,   \

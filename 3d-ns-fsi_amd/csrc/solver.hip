@@ -420,7 +420,7 @@ void Case::applyBCs(T* S, double frac) {
     // immersed body (filter first, then ghosts; src/Boundary.cpp:431-456)
     const double* tdev = &scal->t;
     const double* dtdev = &scal->dt;
-    for (int q = 0; q < 6; ++q) launchEdgeFace<T>(stream, L, P, faces[q], S, tdev, dtdev, frac);
+    launchEdgeFaces<T>(stream, L, P, faces, S, tdev, dtdev, frac);
     if (B.type == NS_BODY_NONE) return;
     launchFilterCompute<T>(stream, L, W, filterList, nFilter, S, (T*)ftmp, (T*)rhoPre, (T*)rhoEPre);
     launchFilterWrite<T>(stream, L, filterList, nFilter, S, (const T*)ftmp);
