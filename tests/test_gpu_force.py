@@ -5,10 +5,15 @@ momentum right-hand sides over the FluidInterior nodes, so the force on the body
 sum times the node volume. The oracle's slopes are the reference's per-node expression
 (computeRK4slopes, src/Solver.cpp:170-418), not the face-flux form the device kernel sums: a dropped
 term, a wrong sign or a wrong face orientation in the device kernel fails this comparison.
+
+A second check is physical: the time-mean drag coefficient of a sphere at Re_D = 20 against the
+Schiller-Naumann correlation (loose bounds for confinement), its agreement between two grids, and zero
+lateral force on the mirror-symmetric case.
 """
 import numpy as np
 import pytest
 
+import cases
 import nsfsi
 import oracle
 
@@ -87,3 +92,45 @@ def test_force_option_leaves_state_unchanged():
             assert s.getForceHistory().shape == (0, 4)
         s.close()
     assert np.array_equal(out[0], out[1])
+
+
+def low_re_sphere(N):
+    """Sphere r = 0.5 at the centre of [0,4]^3 (slip walls in y and z, inlet / outlet in x), Re_D = 20,
+    M = 0.3, unperturbed freestream start with a short inlet ramp."""
+    cfg = cases._base(N, 4.0, 20.0, 0.3, 1.0, 8000, body_type="sphere", body_centre=(2.0, 2.0, 2.0), body_radius=0.5,
+                      ic_type="freestream_gaussian", ic_amplitude=0.0, ic_seed=1, precision="fp64")
+    cfg.inlet_ramp_time = 0.5
+    return cfg
+
+
+def mean_drag(N, steps=8000):
+    cfg = low_re_sphere(N)
+    print(f"low-Re drag run N={N} steps={steps}", flush=True)
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    s.setOption("body_force", 1)
+    s.setOption("graph", 1)
+    s.marchTimeStep(steps)
+    cd = s.dragCoefficient()
+    f = s.getForceHistory()
+    s.close()
+    h = steps // 2
+    return cd[h:].mean(), np.abs(f[h:, 2:]).max() / np.abs(f[h:, 1]).max()
+
+
+def test_drag_low_re_sphere_physical():
+    """Drag of a sphere at Re_D = 20 after about 100 time units, averaged over the second half of the run
+    (the reference's inlet and outlet reflect sound, so C_D oscillates about its mean by +-15 %). The
+    unconfined correlation of Schiller and Naumann gives 24/Re (1 + 0.15 Re^0.687) = 2.61; the slip walls
+    two diameters from the centre and the compressible start raise it. Bounds: within [0.85, 1.6] x 2.61,
+    the 40^3 and 48^3 grids (r = 5 and 6 spacings) within 5 % of each other, and the lateral forces zero to
+    roundoff (the case is mirror-symmetric in y and z)."""
+    re = 20.0
+    sn = 24.0 / re * (1 + 0.15 * re ** 0.687)
+    cd40, lat40 = mean_drag(40)
+    cd48, lat48 = mean_drag(48)
+    print(f"C_D 40^3 {cd40:.4f} 48^3 {cd48:.4f} (Schiller-Naumann {sn:.3f}); lateral/axial {lat40:.1e} {lat48:.1e}")
+    for cd in (cd40, cd48):
+        assert 0.85 * sn <= cd <= 1.6 * sn
+    assert abs(cd40 - cd48) <= 0.05 * cd48
+    assert lat40 <= 1e-10 and lat48 <= 1e-10
