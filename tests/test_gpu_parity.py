@@ -65,6 +65,13 @@ CLASSIFY_CASES = {
         body_centre=(1.31, 1.27, 1.33), body_radius=0.61, stop_criterion="timesteps", stop_time_level=5,
         filter_intervals=[0], filter_change_times=[], ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=7,
         M_0=0.3, Re=300.0, conv_stability_limit=0.5),
+    # the sphere straddles the seam between x-segments of the device layout (nodes 61 | 62): ghost boxes,
+    # filter stencils and interpolation corners cross the padding slots between segments
+    "sphere_seam100": lambda: nsfsi.make_config(
+        NI=100, NJ=30, NK=28, L_x=4.0, L_y=1.2, L_z=1.2, bc=cases.SIDE_WALLS, body_type="sphere",
+        body_centre=(2.485, 0.61, 0.59), body_radius=0.3, stop_criterion="timesteps", stop_time_level=5,
+        filter_intervals=[0], filter_change_times=[], ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=11,
+        M_0=0.3, Re=300.0, conv_stability_limit=0.5),
 }
 
 
@@ -89,7 +96,7 @@ def test_classification_bit_exact(name):
     assert np.array_equal(s.mesh.positionOffset, of)
 
 
-@pytest.mark.parametrize("name", ["sphere24", "sphere48", "cylinder_z64", "sphere_offcentre33"])
+@pytest.mark.parametrize("name", ["sphere24", "sphere48", "cylinder_z64", "sphere_offcentre33", "sphere_seam100"])
 def test_initial_state_bit_exact(name):
     cfg = CLASSIFY_CASES[name]()
     s, o = _pair(cfg)
@@ -99,7 +106,7 @@ def test_initial_state_bit_exact(name):
 
 @pytest.mark.parametrize("kernel", [1, 0], ids=["fused", "unfused"])
 @pytest.mark.parametrize("name,steps", [("sphere24", 20), ("sphere48", 20), ("cylinder_z64", 10),
-                                        ("sphere_offcentre33", 15)])
+                                        ("sphere_offcentre33", 15), ("sphere_seam100", 15)])
 def test_trajectory_fp64(name, steps, kernel):
     cfg = CLASSIFY_CASES[name]()
     s, o = _pair(cfg)
