@@ -119,6 +119,25 @@ void Batch::init() {
         faceTotal[f] = fstart[f][n];
         hipCheck(hipMemcpy(devFaceStart[f], fstart[f].data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice), "copy");
     }
+    // faces in application order, grouped greedily: a face joins the current launch when it is independent
+    // of every face already in it in every case
+    faceGroups.clear();
+    EdgeFaceListB g{};
+    for (int f = 0; f < 6; ++f) {
+        if (faceTotal[f] == 0) continue;
+        bool ok = true;
+        for (int m = 0; m < g.n && ok; ++m)
+            for (int c = 0; c < n && ok; ++c) ok = edgeFacesIndependent(views[c].L, views[c].faces[g.face[m]], views[c].faces[f]);
+        if (!ok) {
+            faceGroups.push_back(g);
+            g = EdgeFaceListB{};
+        }
+        g.face[g.n] = f;
+        g.start[g.n] = devFaceStart[f];
+        g.off[g.n + 1] = g.off[g.n] + faceTotal[f];
+        ++g.n;
+    }
+    if (g.n > 0) faceGroups.push_back(g);
     devFilterStart = (int64_t*)alloc((n + 1) * sizeof(int64_t));
     devGhostStart = (int64_t*)alloc((n + 1) * sizeof(int64_t));
     devBBStart = (int*)alloc((n + 1) * sizeof(int));
@@ -174,8 +193,7 @@ void Batch::buildStageTables() {
 
 void Batch::applyBCs(int role, double frac) {
     const int n = (int)cases.size();
-    for (int f = 0; f < 6; ++f)
-        launchEdgeFaceBatched<float>(stream, devViews, devFaceStart[f], faceTotal[f], n, f, role, parity, frac);
+    for (const EdgeFaceListB& g : faceGroups) launchEdgeFacesBatched<float>(stream, devViews, g, n, role, parity, frac);
     launchFilterComputeBatched<float>(stream, devViews, devFilterStart, filterTotal, n, role, parity);
     launchFilterWriteBatched<float>(stream, devViews, devFilterStart, filterTotal, n, role, parity);
     launchGhostApplyBatched<float>(stream, devViews, devGhostStart, ghostTotal, n, role, parity);

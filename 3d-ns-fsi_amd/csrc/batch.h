@@ -63,9 +63,18 @@ __device__ __forceinline__ int findCase(const I* __restrict__ start, int ncases,
 }
 
 // ---- batched launchers (one launch for all cases; *Start arrays are device prefix sums over cases)
+// Faces of every case launched together (faces that are independent in every case): member m is face
+// face[m] of each case, its nodes over the cases given by the device prefix sums start[m], and it takes
+// the thread range [off[m], off[m+1]).
+struct EdgeFaceListB {
+    int face[6];
+    const int64_t* start[6];
+    int64_t off[7];
+    int n;
+};
 template <class T>
-void launchEdgeFaceBatched(hipStream_t s, const CaseView<T>* cv, const int64_t* start, int64_t total, int ncases,
-                           int face, int role, int parity, double frac);
+void launchEdgeFacesBatched(hipStream_t s, const CaseView<T>* cv, const EdgeFaceListB& fl, int ncases, int role, int parity,
+                            double frac);
 template <class T>
 void launchFilterComputeBatched(hipStream_t s, const CaseView<T>* cv, const int64_t* start, int64_t total, int ncases,
                                 int role, int parity);

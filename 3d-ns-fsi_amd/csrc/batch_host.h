@@ -49,6 +49,7 @@ private:
     int* devStageStart = nullptr;
     int64_t* devFaceStart[6] = {};
     int64_t faceTotal[6] = {};
+    std::vector<EdgeFaceListB> faceGroups;  // faces launched together, in application order
     int64_t* devFilterStart = nullptr;
     int64_t* devGhostStart = nullptr;
     int64_t filterTotal = 0, ghostTotal = 0;

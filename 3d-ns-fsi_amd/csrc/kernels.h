@@ -51,6 +51,9 @@ void launchGhostGeometry(hipStream_t s, const Layout& L, const uint8_t* cls, con
                          const int64_t* ghosts, int64_t n, const int32_t* gmap, GhostTables G, int32_t* error);
 
 // ---- boundary conditions on a state buffer (mesh.hip)
+// Whether two edge faces can be applied in one launch: disjoint owned boxes and neither reads a node the
+// other writes (the planes one and two in along the normal, or a periodic face's source plane).
+bool edgeFacesIndependent(const Layout& L, const EdgeFace& x, const EdgeFace& y);
 // All six edge faces in application order, independent faces sharing one launch (same result as
 // launchEdgeFace face by face).
 template <class T>

@@ -405,14 +405,20 @@ __global__ void k_edge_faces(Layout L, Params P, EdgeFaceGroup g, T* S, const do
     edgeFaceNode(L, P, g.f[q], S, ts, (lb % g.nbx[q]) * 64 + threadIdx.x, lb / g.nbx[q]);
 }
 
+// Several faces of every case in one launch: entries [off[m], off[m+1]) of the thread range belong to face
+// face[m], whose nodes over the cases follow the prefix sums start[m].
 template <class T>
-__global__ void k_edge_face_batched(const CaseView<T>* __restrict__ cv, const int64_t* __restrict__ start, int ncases,
-                                    int face, int role, int parity, double frac) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= start[ncases]) return;
+__global__ void k_edge_faces_batched(const CaseView<T>* __restrict__ cv, EdgeFaceListB fl, int ncases, int role,
+                                     int parity, double frac) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fl.off[fl.n]) return;
+    int m = 0;
+    while (m + 1 < fl.n && t >= fl.off[m + 1]) ++m;
+    const int64_t* start = fl.start[m];
+    const int64_t q = t - fl.off[m];
     const int c = findCase(start, ncases, q);
     const CaseView<T>& v = cv[c];
-    const EdgeFace& f = v.faces[face];
+    const EdgeFace& f = v.faces[fl.face[m]];
     const int ax1 = f.axis == 0 ? 1 : 0;
     const int lo1 = ax1 == 0 ? f.i0 : f.j0, hi1 = ax1 == 0 ? f.i1 : f.j1;
     const int64_t r = q - start[c];
@@ -650,11 +656,11 @@ Box readBox(const Layout& L, const EdgeFace& f) {
     return r;
 }
 
-bool independent(const Layout& L, const EdgeFace& x, const EdgeFace& y) {
+}  // namespace
+
+bool edgeFacesIndependent(const Layout& L, const EdgeFace& x, const EdgeFace& y) {
     return !readBox(L, x).meets(ownedBox(y)) && !readBox(L, y).meets(ownedBox(x)) && !ownedBox(x).meets(ownedBox(y));
 }
-
-}  // namespace
 
 template <class T>
 void launchEdgeFaces(hipStream_t s, const Layout& L, const Params& P, const EdgeFace faces[6], T* S, const double* tdev,
@@ -671,7 +677,7 @@ void launchEdgeFaces(hipStream_t s, const Layout& L, const Params& P, const Edge
         const EdgeFace& f = faces[q];
         if (ownedBox(f).empty()) continue;
         bool ok = true;
-        for (int r = 0; r < g.n; ++r) ok = ok && independent(L, g.f[r], f);
+        for (int r = 0; r < g.n; ++r) ok = ok && edgeFacesIndependent(L, g.f[r], f);
         if (!ok) flush();
         const int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
         const Box b = ownedBox(f);
@@ -704,10 +710,10 @@ void launchGhostApply(hipStream_t s, const Layout& L, const Params& P, const Gho
 }
 
 template <class T>
-void launchEdgeFaceBatched(hipStream_t s, const CaseView<T>* cv, const int64_t* start, int64_t total, int ncases, int face,
-                           int role, int parity, double frac) {
-    if (total == 0) return;
-    hipLaunchKernelGGL(k_edge_face_batched<T>, dim3(blocks(total, 128)), dim3(128), 0, s, cv, start, ncases, face, role,
+void launchEdgeFacesBatched(hipStream_t s, const CaseView<T>* cv, const EdgeFaceListB& fl, int ncases, int role, int parity,
+                            double frac) {
+    if (fl.n == 0 || fl.off[fl.n] == 0) return;
+    hipLaunchKernelGGL(k_edge_faces_batched<T>, dim3(blocks(fl.off[fl.n], 128)), dim3(128), 0, s, cv, fl, ncases, role,
                        parity, frac);
 }
 
@@ -736,8 +742,7 @@ void launchGhostApplyBatched(hipStream_t s, const CaseView<T>* cv, const int64_t
 }
 
 #define NS_INST(T)                                                                                                           \
-    template void launchEdgeFaceBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int, int,     \
-                                           double);                                                                           \
+    template void launchEdgeFacesBatched<T>(hipStream_t, const CaseView<T>*, const EdgeFaceListB&, int, int, int, double);   \
     template void launchFilterComputeBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);    \
     template void launchFilterWriteBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);      \
     template void launchGhostApplyBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);       \
