@@ -387,10 +387,14 @@ __global__ void k_edge_face(Layout L, Params P, EdgeFace f, T* S, const double* 
 
 // Several edge faces in one launch (faces that neither write what another of them reads): face q takes
 // the blocks start[q] .. start[q+1]-1, laid out as its own (ceil(n1/64), n2) grid.
+// rowSlots: the face's first in-plane axis is x and it owns whole rows; its threads then run over the slots
+// of a row, and the padding slots between x-segments are written too (zeros, never read), so the face's
+// lines are written whole.
 struct EdgeFaceGroup {
     EdgeFace f[6];
     int start[7];
     int nbx[6];
+    int rowSlots[6];
     int n;
 };
 
@@ -401,8 +405,23 @@ __global__ void k_edge_faces(Layout L, Params P, EdgeFaceGroup g, T* S, const do
     int q = 0;
     while (q + 1 < g.n && b >= g.start[q + 1]) ++q;
     const int lb = b - g.start[q];
+    const int a = (lb % g.nbx[q]) * 64 + threadIdx.x, bb = lb / g.nbx[q];
     const double ts = *tdev + frac * *dtdev;
-    edgeFaceNode(L, P, g.f[q], S, ts, (lb % g.nbx[q]) * 64 + threadIdx.x, lb / g.nbx[q]);
+    const EdgeFace& f = g.f[q];
+    if (g.rowSlots[q]) {
+        if (a >= L.pitch) return;
+        const int r = a % Layout::kSegSlots;
+        const int i = (a / Layout::kSegSlots) * Layout::kSegNodes + r - 1;
+        if (r == 0 || r == Layout::kSegSlots - 1 || i >= L.NI) {
+            const int j = f.axis == 1 ? f.j0 : f.j0 + bb, k = f.axis == 1 ? f.k0 + bb : f.k0;
+            const size_t o = ((size_t)k * L.NJ + j) * L.pitch + a;
+            for (int c = 0; c < 5; ++c) S[o + c * L.fstride] = (T)0;
+            return;
+        }
+        edgeFaceNode(L, P, f, S, ts, i, bb);
+    } else {
+        edgeFaceNode(L, P, f, S, ts, a, bb);
+    }
 }
 
 // Several faces of every case in one launch: entries [off[m], off[m+1]) of the thread range belong to face
@@ -681,8 +700,10 @@ void launchEdgeFaces(hipStream_t s, const Layout& L, const Params& P, const Edge
         if (!ok) flush();
         const int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
         const Box b = ownedBox(f);
-        const int n1 = b.hi[ax1] - b.lo[ax1] + 1, n2 = b.hi[ax2] - b.lo[ax2] + 1;
+        const bool rows = ax1 == 0 && b.lo[0] == 0 && b.hi[0] == L.NI - 1;
+        const int n1 = rows ? L.pitch : b.hi[ax1] - b.lo[ax1] + 1, n2 = b.hi[ax2] - b.lo[ax2] + 1;
         g.f[g.n] = f;
+        g.rowSlots[g.n] = rows;
         g.nbx[g.n] = (n1 + 63) / 64;
         g.start[g.n + 1] = g.start[g.n] + g.nbx[g.n] * n2;
         ++g.n;
