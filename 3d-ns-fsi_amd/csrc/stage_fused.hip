@@ -12,13 +12,15 @@
  * the reference expression (same stencil, same coefficients); only the rounding differs. Each face flux
  * is computed once and shared by the two nodes of the face.
  *
- * Tiling. A block is 64 x (TY + 2) threads: one wavefront per x-row. Lanes 1..62 own output columns,
- * lanes 0 and 63 are the x-halo; rows 1..TY own output rows, rows 0 and TY+1 the y-halo. The block
- * streams through a chunk of z-planes: the own column's neighbours in z live in registers, x-neighbours
- * come by cross-lane shuffle, y-neighbours through LDS. Primitive variables are recomputed from the
- * conserved state; at immersed-filter nodes from the pre-filter density and energy (the reference keeps
- * primitives pre-filter there). Blocks are mapped so that each XCD sweeps a contiguous slab (its L2 then
- * holds the halo of its neighbours).
+ * Tiling. A block is 64 x (TY + 2) threads: one wavefront per x-row. The x-tiles are the layout's
+ * x-segments (62 nodes in 64 slots): lanes 1..62 own the segment's nodes, lanes 0 and 63 are the x-halo
+ * (they read the neighbouring segments' edge nodes); rows 1..TY own output rows, rows 0 and TY+1 the
+ * y-halo. The block streams through a chunk of z-planes: the own column's neighbours in z live in
+ * registers, x-neighbours come by cross-lane shuffle, y-neighbours through LDS. Primitive variables are
+ * recomputed from the conserved state; at immersed-filter nodes from the pre-filter density and energy
+ * (the reference keeps primitives pre-filter there). Each output wave stores its whole segment, padding
+ * slots included, with streaming stores, so every line it writes is written whole. Blocks are mapped so
+ * that each XCD sweeps a contiguous slab (its L2 then holds the halo of its neighbours).
  *
  * Multiply-adds are fused only inside one source expression (contract(on), not the compiler's
  * cross-statement fusion), so the arithmetic of a node does not depend on which block, tile shape or
@@ -366,7 +368,7 @@ __device__ __forceinline__ void readFaceNb(const Smem<T, TY>& sm, int r, int lan
     nb.Tt = sm.face[9][r][lane];
 }
 
-// One tile (block lb of the case described by a): the whole stage for 62 x TY output columns over one
+// One tile (block lb of the case described by a): the whole stage for one x-segment x TY output rows over one
 // z-chunk.
 template <class T, int STAGE, int TY, bool SCAN, bool KAHAN, bool BUF>
 __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, Smem<T, TY>& sm,
