@@ -4,7 +4,7 @@ J = mean C_D over time levels n0..n1; the adjoint differentiates the run at its 
 finite-difference evaluation runs at the same prescribed step sequence (ns_set_dt_schedule). Directions
 are random over all conserved fields of all nodes of the initial state, over the inlet profile, and Re.
 Central differences with eps = 1e-4 (FP64) have a truncation error far below the 1e-6 relative bar
-used here at these sizes (the north-star bar is 1e-4).
+used here at these sizes (the bar the gradients must meet is 1e-4).
 """
 import numpy as np
 import pytest
