@@ -51,7 +51,7 @@ def test_autotune_leaves_state_unchanged():
 def test_fp32_global_access_bitwise(kahan):
     """FP32 states beyond 4 GiB use per-field global addressing instead of buffer resources; forcing that
     form on a small case gives the same state bit for bit (compensated and plain combine)."""
-    cfg = cases.sphere48(N=41, steps=6, precision="fp32")
+    cfg = cases.sphere48(N=70, steps=6, precision="fp32")  # two x-segments (62 + 8 nodes)
     cfg.kahan = kahan
     ref = run(cfg, 6, tile_ty=6, tile_kc=16)
     U = run(cfg, 6, tile_ty=6, tile_kc=16, global_access=1)
