@@ -19,9 +19,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "3d-ns-fsi_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per stage launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy) for one
-# workload form (workload, precision, combine) at one tile; another form reports traffic as null.
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_fused_fp32_sphere512_traffic.json")
+# HBM bytes per stage launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy), one file
+# per profiled workload form (workload, precision, combine) at one tile; other forms report traffic as null.
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f"k_stage_fused_{p}_sphere512_traffic.json") for p in ("fp32", "fp64")]
 VALUES_PER_NODE_STEP = 80  # 3-register RK4: stage 1 reads 5 writes 10, stages 2-3 read 15 write 10, stage 4 reads 10 writes 5
 
 
@@ -51,14 +51,15 @@ def cpu_baseline(workload, precision_unused=None):
 
 def measured_traffic(workload, precision, kahan, kernel):
     """(bytes per stage launch, tile it was measured at) for this workload form, or (None, None)."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None, None
-    if (kernel == "fused" and t.get("workload") == workload and t.get("precision") == precision
-            and bool(t.get("compensated_combine")) == bool(kahan)):
-        return t.get("traffic_bytes_per_stage_launch"), list(t.get("tile", []))
+    for path in TRAFFIC_FILES:
+        try:
+            with open(path) as f:
+                t = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if (kernel == "fused" and t.get("workload") == workload and t.get("precision") == precision
+                and bool(t.get("compensated_combine")) == bool(kahan)):
+            return t.get("traffic_bytes_per_stage_launch"), list(t.get("tile", []))
     return None, None
 
 
