@@ -35,7 +35,9 @@ Layout makeLayout(int NI, int NJ, int NK, size_t eb) {
     L.NI = NI;
     L.NJ = NJ;
     L.NK = NK;
-    size_t rowElems = 128 / eb;
+    // FP32 rows hold whole segments (256 B), so the bfloat16 compensation carry, stored at half the byte
+    // offsets, also starts every row on a 128-byte line; FP64 rows are padded to 128 B
+    const size_t rowElems = eb == 4 ? (size_t)Layout::kSegSlots : 128 / eb;
     const size_t slots = (size_t)Layout::col(NI - 1) + 1;  // through the last node's slot
     L.pitch = (int)((slots + rowElems - 1) / rowElems * rowElems);
     L.plane = (size_t)NJ * L.pitch;

@@ -396,8 +396,15 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     // the own slot: lane l writes slot l of segment bx (lanes 0 and 63 the padding slots)
     const uint32_t colo = slotBytes<T>(L, Layout::kSegSlots * bx + lane, j);
     const uint32_t planeB = (uint32_t)(L.plane * sizeof(T));
-    // padding slots of this segment that exist in the row (slot 63 only if the next segment does)
-    const bool padLane = lane == 0 || (lane == 63 && i < L.NI);
+    // Slots without a node that this wave writes (values never read): the segment's padding slots and, in
+    // FP32, in the row's last segment, the slots past the last node up to the end of its 128-byte line
+    // (for the bfloat16 carry, whose line spans a whole segment, up to the segment's end), so that no line
+    // of the row is written in part. (FP64 ran 0.5 % slower with its last line completed on 512-node rows,
+    // whose last line holds one node and fifteen slots past it.)
+    const int slot = Layout::kSegSlots * bx + lane, endSlot = Layout::col(L.NI - 1) + 1;
+    const bool padLane = lane == 0 || lane == 63 || i >= L.NI;
+    const bool padLine = padLane && slot < (sizeof(T) == 4 ? (endSlot + 31) / 32 * 32 : endSlot);
+    const bool padSeg = padLane && slot < min(L.pitch, (endSlot + 63) / 64 * 64);
 
     // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
     Pt<T> qm, q0, q1;
@@ -510,7 +517,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
             T cur[5] = {q0.rho, q0.ru, q0.rv, q0.rw, q0.E};
             T vS[5], vA[5];
             uint16_t vC[5];
-            bool wS = padLane, wA = padLane, wC = padLane;
+            bool wS = padLine, wA = padLine, wC = padSeg;
             for (int q = 0; q < 5; ++q) {
                 vS[q] = vA[q] = cur[q];
                 vC[q] = 0;
