@@ -15,7 +15,9 @@
  * Tiling. A block is 64 x (TY + 2) threads: one wavefront per x-row. The x-tiles are the layout's
  * x-segments (62 nodes in 64 slots): lanes 1..62 own the segment's nodes, lanes 0 and 63 are the x-halo
  * (they read the neighbouring segments' edge nodes); rows 1..TY own output rows, rows 0 and TY+1 the
- * y-halo. The block streams through a chunk of z-planes: the own column's neighbours in z live in
+ * y-halo. A row's last segment, when it holds at most 30 nodes, is tiled at half width: each half of the
+ * wave takes it in one of two y-bands, so a 512-node row costs 8.5 waves instead of 9 and a 64-node row
+ * 1.5 instead of 2. The block streams through a chunk of z-planes: the own column's neighbours in z live in
  * registers, x-neighbours come by cross-lane shuffle, y-neighbours through LDS. Primitive variables are
  * recomputed from the conserved state; at immersed-filter nodes from the pre-filter density and energy
  * (the reference keeps primitives pre-filter there). Each output wave stores its whole segment, padding
@@ -87,6 +89,7 @@ struct FusedArgs {
     const T* rhoEPre;
     const double* dtdev;
     int nbx, nby, nbz, kc;   // tiles in x, y, z and planes per z-chunk
+    int tailHalf;            // the last x-segment holds at most 30 nodes: its tiles are half-width
     int nblocks;
     // stage 4 only: per-block partials of the end-of-step scans over interior, non-filter nodes
     // (slots: 3 spectral-radius maxima, viscous maximum, 5 sums of squared change, non-finite count)
@@ -368,21 +371,58 @@ __device__ __forceinline__ void readFaceNb(const Smem<T, TY>& sm, int r, int lan
     nb.Tt = sm.face[9][r][lane];
 }
 
+// Tile lb of a z-chunk order: (x-segment, y-band, z-chunk) and whether the tile is half-width.
+struct TileIdx {
+    int bx, by, bz;
+    bool half;
+};
+
+// Tiles in x-segment-fastest order. With half-width tail tiles the y-bands go in pairs: the pair's full
+// tiles (band 2m, then band 2m+1, segments 0..nbx-2 each), then one tile of the last segment holding both
+// bands (the last pair of an odd band count has one band).
+__device__ __forceinline__ TileIdx tileOf(int lb, int nbx, int nby, bool tailHalf) {
+    TileIdx t;
+    if (!tailHalf) {
+        t.bx = lb % nbx;
+        t.by = (lb / nbx) % nby;
+        t.bz = lb / (nbx * nby);
+        t.half = false;
+        return t;
+    }
+    const int nf = nbx - 1, perPair = 2 * nf + 1, perChunk = nf * nby + (nby + 1) / 2;
+    t.bz = lb / perChunk;
+    const int r0 = lb % perChunk, m = r0 / perPair, r = r0 % perPair;
+    const int nfull = (nby & 1) && m == nby / 2 ? nf : 2 * nf;
+    if (r < nfull) {
+        t.bx = r % nf;
+        t.by = 2 * m + r / nf;
+        t.half = false;
+    } else {
+        t.bx = nf;
+        t.by = 2 * m;
+        t.half = true;
+    }
+    return t;
+}
+
 // One tile (block lb of the case described by a): the whole stage for one x-segment x TY output rows over one
-// z-chunk.
+// z-chunk. When a row's last segment holds at most 30 nodes, its tiles are half-width: lanes 0..31 take the
+// segment in y-band by, lanes 32..63 in band by+1, each half with its own halo lanes (sub-lane 0 and the
+// sub-lanes past the last node), so a wave shift across the halves only moves values between halo lanes.
 template <class T, int STAGE, int TY, bool SCAN, bool KAHAN, bool BUF>
 __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, Smem<T, TY>& sm,
                                           double (&wred)[TY + 2][10]) {
-    const int bx = lb % a.nbx;
-    const int by = (lb / a.nbx) % a.nby;
-    const int bz = lb / (a.nbx * a.nby);
+    const TileIdx tl = tileOf(lb, a.nbx, a.nby, a.tailHalf);
+    const int bx = tl.bx, bz = tl.bz;
     const int lane = threadIdx.x, row = threadIdx.y;
+    const int sl = tl.half ? lane & 31 : lane;            // lane within the (half-)tile
+    const int by = tl.half ? tl.by + (lane >> 5) : tl.by;  // y-band of this lane
     const Layout& L = a.L;
-    const int i = bx * 62 + lane - 1;
+    const int i = bx * 62 + sl - 1;
     const int j = by * TY + row - 1;
     const int kb = bz * a.kc, ke = min(L.NK, kb + a.kc);
     const bool outRow = row >= 1 && row <= TY && j < L.NJ;
-    const bool outLane = lane >= 1 && lane <= 62 && i < L.NI;
+    const bool outLane = sl >= 1 && sl <= 62 && i < L.NI;
     const Coef<T>& c = a.c;
     const double dtd = *a.dtdev;
     const bool rhoFiltered = STAGE == 1 && a.fFlag && *a.fFlag;
@@ -393,18 +433,21 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     // is segment bx of the layout, so the halo lanes read the neighbouring segments
     const int ic = min(max(i, 0), L.NI - 1), jc = min(max(j, 0), L.NJ - 1);
     const uint32_t colc = slotBytes<T>(L, Layout::col(ic), jc);
-    // the own slot: lane l writes slot l of segment bx (lanes 0 and 63 the padding slots)
-    const uint32_t colo = slotBytes<T>(L, Layout::kSegSlots * bx + lane, j);
+    // the own slot: sub-lane l writes slot l of segment bx (sub-lanes 0 and 63 the padding slots)
+    const uint32_t colo = slotBytes<T>(L, Layout::kSegSlots * bx + sl, j);
     const uint32_t planeB = (uint32_t)(L.plane * sizeof(T));
     // Slots without a node that this wave writes (values never read): the segment's padding slots and, in
     // FP32, in the row's last segment, the slots past the last node up to the end of its 128-byte line
-    // (for the bfloat16 carry, whose line spans a whole segment, up to the segment's end), so that no line
-    // of the row is written in part. (FP64 ran 0.5 % slower with its last line completed on 512-node rows,
-    // whose last line holds one node and fifteen slots past it.)
-    const int slot = Layout::kSegSlots * bx + lane, endSlot = Layout::col(L.NI - 1) + 1;
-    const bool padLane = lane == 0 || lane == 63 || i >= L.NI;
+    // (for the bfloat16 carry, whose line spans a whole segment, up to the segment's end; a half-width
+    // tile writes the carry's upper half with a second store), so that no line of the row is written in
+    // part. (FP64 ran 0.5 % slower with its last line completed on 512-node rows, whose last line holds
+    // one node and fifteen slots past it.)
+    const int slot = Layout::kSegSlots * bx + sl, endSlot = Layout::col(L.NI - 1) + 1;
+    const bool padLane = sl == 0 || sl == 63 || i >= L.NI;
     const bool padLine = padLane && slot < (sizeof(T) == 4 ? (endSlot + 31) / 32 * 32 : endSlot);
-    const bool padSeg = padLane && slot < min(L.pitch, (endSlot + 63) / 64 * 64);
+    const int segEnd = min(L.pitch, (endSlot + 63) / 64 * 64);
+    const bool padSeg = padLane && slot < segEnd;
+    const bool upperSeg = tl.half && slot + 32 < segEnd;  // carry slot + 32 of a half-width tile
 
     // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
     Pt<T> qm, q0, q1;
@@ -604,6 +647,8 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                 for (int q = 0; q < 5; ++q) m.stA(q, bo, vA[q]);
             if (STAGE == 4 && KAHAN && wC)
                 for (int q = 0; q < 5; ++q) m.stComp(q, bo, vC[q]);
+            if (STAGE == 4 && KAHAN && upperSeg)
+                for (int q = 0; q < 5; ++q) m.stComp(q, bo + 32 * (uint32_t)sizeof(T), 0);
         }
         // rotate the column
         umz = q0.u;
@@ -880,7 +925,8 @@ void fillArgs(FusedArgs<T>& a, int stage, const Layout& L, const Params& P, cons
     a.nbx = (L.NI + 61) / 62;
     a.nby = (L.NJ + ty - 1) / ty;
     a.nbz = (L.NK + a.kc - 1) / a.kc;
-    a.nblocks = a.nbx * a.nby * a.nbz;
+    a.tailHalf = L.NI - Layout::kSegNodes * (a.nbx - 1) <= 30;
+    a.nblocks = ((a.nbx - a.tailHalf) * a.nby + (a.tailHalf ? (a.nby + 1) / 2 : 0)) * a.nbz;
 }
 
 }  // namespace

@@ -24,10 +24,13 @@ def run(cfg, steps, **opts):
     return U
 
 
+@pytest.mark.parametrize("N", [41, 64])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_tile_shape_invariance(prec):
-    # 41 planes: the last z-chunk of every tile height has an odd plane count (9 or 41)
-    cfg = cases.sphere48(N=41, steps=4, precision=prec)
+def test_tile_shape_invariance(prec, N):
+    # N = 41: one x-segment; the last z-chunk of every tile height has an odd plane count (9 or 41).
+    # N = 64: the last x-segment holds 2 nodes, so its tiles are half-width (two y-bands per wave), with an
+    # odd y-band count at 6 and 14 rows (the last band pair has one band).
+    cfg = cases.sphere48(N=N, steps=4, precision=prec)
     ref = run(cfg, 4, tile_ty=8, tile_kc=64)
     for ty in (4, 6, 14):
         for kc in (12, 16, 20, 32, 128):
