@@ -76,7 +76,8 @@ void Batch::init() {
             int64_t cnt = std::max<int64_t>(0, (int64_t)(e.i1 - e.i0 + 1)) * std::max<int64_t>(0, (int64_t)(e.j1 - e.j0 + 1)) *
                           std::max<int64_t>(0, (int64_t)(e.k1 - e.k0 + 1));
             v.faceStart[f + 1] = v.faceStart[f] + cnt;
-            fstart[f][c + 1] = fstart[f][c] + cnt;
+            // the edge-face pass skips the outlet face that the case's stage kernel applies itself
+            fstart[f][c + 1] = fstart[f][c] + (k.stageOutlet() == &k.faces[f] ? 0 : cnt);
         }
         v.cls = k.cls;
         v.buf[0] = (float*)k.U;
@@ -177,7 +178,7 @@ void Batch::buildStageTables() {
                 int nb = buildFusedArgs(h.data() + c * ab, st, k.L, k.P, k.W, in[st - 1], U, out[st - 1], A, k.cls,
                                         (const float*)k.rhoPre, (const float*)k.rhoEPre, &k.scal->dt, tile,
                                         st == 4 ? k.partials : nullptr, k.cfg.conv_history > 0,
-                                        kahan ? k.comp : nullptr);
+                                        kahan ? k.comp : nullptr, k.stageOutlet());
                 start[c + 1] = start[c] + nb;
                 hostViews[c].nStage4 = nb;
                 const size_t need = (size_t)nb + hostViews[c].nBoundaryBlocks + hostViews[c].nInactiveBlocks;

@@ -71,6 +71,12 @@ public:
     Window W{};
     BodyDesc B{};
     EdgeFace faces[6]{};  // application order (reverse of the claim order)
+    // faces[fusedOutlet] (or -1): the outlet face that the fused stage kernel applies itself; bcFaces are the
+    // faces with that one left out, for the edge-face pass after a fused stage
+    int fusedOutlet = -1;
+    EdgeFace bcFaces[6]{};
+    bool fuseOutletOpt = true;  // option "fused_outlet"
+    const EdgeFace* stageOutlet() const;  // the face the fused stage kernel applies, or null
 
     // classification results
     uint8_t* cls = nullptr;
@@ -133,7 +139,7 @@ private:
     template <class T> void hostInitialField(std::vector<T>& h) const;
     template <class T> void uploadState(const std::vector<T>& h);
     template <class T> void computeDt(int mode);
-    template <class T> void applyBCs(T* S, double frac);
+    template <class T> void applyBCs(T* S, double frac, bool outletDone);
     template <class T> void stepOnce();
     std::vector<void*> allocations;
     StageTape* tapeNow = nullptr;

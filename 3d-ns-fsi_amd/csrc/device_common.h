@@ -57,6 +57,26 @@ __device__ __forceinline__ void deriveCons(const Eos<T>& e, T u, T v, T w, T p, 
     rE = p / e.gm1 + m / (T)2 * (u * u + v * v + w * w);
 }
 
+// Outlet node values (OutletBoundary::applyBoundaryCondition, src/Boundary.cpp:304-333): density and
+// momenta extrapolated linearly from the nodes one (a) and two (b) in from the face, zero pressure
+// perturbation. Multiply-adds stay unfused here, so the edge-face kernel and the fused stage kernel, which
+// both evaluate it, give the same bits.
+template <class T>
+__device__ __forceinline__ void outletValues(T gm1, const T a[4], const T b[4], T out[5]) {
+#pragma clang fp contract(off)
+    const T r = (T)2 * a[0] - b[0];
+    const T ru = (T)2 * a[1] - b[1];
+    const T rv = (T)2 * a[2] - b[2];
+    const T rw = (T)2 * a[3] - b[3];
+    const T p = 0;
+    const T u = ru / ((T)1 + r), v = rv / ((T)1 + r), w = rw / ((T)1 + r);
+    out[0] = r;
+    out[1] = ru;
+    out[2] = rv;
+    out[3] = rw;
+    out[4] = p / gm1 + ((T)1 + r) / (T)2 * (u * u + v * v + w * w);
+}
+
 // Compensated FP32 combine. The increment form keeps Uⁿ out of the stage accumulator (it holds
 // D = dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4 only), so the one rounding that is relative to |U| happens
 // in U + D at the end of the step. TwoSum (Knuth) gives that rounding error exactly; it is carried to

@@ -352,14 +352,13 @@ __device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, c
         deriveCons(e, uu, vv, ww, p, Tt, rho, ru, rv, rw, rE);
         F.f[0][o] = rho; F.f[1][o] = ru; F.f[2][o] = rv; F.f[3][o] = rw; F.f[4][o] = rE;
     } else if (f.type == NS_BC_OUTLET) {
-        T r = (T)2 * F.f[0][o + s1] - F.f[0][o + s2];
-        T ru = (T)2 * F.f[1][o + s1] - F.f[1][o + s2];
-        T rv = (T)2 * F.f[2][o + s1] - F.f[2][o + s2];
-        T rw = (T)2 * F.f[3][o + s1] - F.f[3][o + s2];
-        T p = 0;
-        T u = ru / ((T)1 + r), v = rv / ((T)1 + r), w = rw / ((T)1 + r);
-        T rE = p / e.gm1 + ((T)1 + r) / (T)2 * (u * u + v * v + w * w);
-        F.f[0][o] = r; F.f[1][o] = ru; F.f[2][o] = rv; F.f[3][o] = rw; F.f[4][o] = rE;
+        T a[4], b[4], v[5];
+        for (int q = 0; q < 4; ++q) {
+            a[q] = F.f[q][o + s1];
+            b[q] = F.f[q][o + s2];
+        }
+        outletValues(e.gm1, a, b, v);
+        for (int q = 0; q < 5; ++q) F.f[q][o] = v[q];
     } else if (f.type == NS_BC_PERIODIC) {
         int cc[3] = {c[0], c[1], c[2]};
         int Nn = f.axis == 0 ? L.NI : (f.axis == 1 ? L.NJ : L.NK);

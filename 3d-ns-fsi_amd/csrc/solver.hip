@@ -205,6 +205,20 @@ void Case::setupBoundaries() {
         pushed[q] = f;
     }
     for (int q = 0; q < 6; ++q) faces[q] = pushed[5 - q];
+    // The outlet face goes into the fused stage kernel when the kernel can evaluate it and it is independent
+    // of every face applied before it: applying it first, inside the stage, then changes nothing.
+    fusedOutlet = -1;
+    for (int q = 0; q < 6; ++q) bcFaces[q] = faces[q];
+    for (int q = 0; q < 6 && fusedOutlet < 0; ++q) {
+        if (!outletFusable(L, faces[q])) continue;
+        bool ok = true;
+        for (int r = 0; r < q; ++r) ok = ok && edgeFacesIndependent(L, faces[r], faces[q]);
+        if (ok) fusedOutlet = q;
+    }
+    if (fusedOutlet >= 0) {  // empty owned box: the edge-face pass skips it
+        bcFaces[fusedOutlet].i0 = 1;
+        bcFaces[fusedOutlet].i1 = 0;
+    }
     // body
     B = BodyDesc{};
     B.type = cfg.body_type;
@@ -417,12 +431,12 @@ void Case::computeDt(int mode) {
 }
 
 template <class T>
-void Case::applyBCs(T* S, double frac) {
+void Case::applyBCs(T* S, double frac, bool outletDone) {
     // Mesh::applyAllBoundaryConditions, src/Mesh.cpp:155-167: edge faces in reverse claim order, then the
     // immersed body (filter first, then ghosts; src/Boundary.cpp:431-456)
     const double* tdev = &scal->t;
     const double* dtdev = &scal->dt;
-    launchEdgeFaces<T>(stream, L, P, faces, S, tdev, dtdev, frac);
+    launchEdgeFaces<T>(stream, L, P, outletDone ? bcFaces : faces, S, tdev, dtdev, frac);
     if (B.type == NS_BODY_NONE) return;
     launchFilterCompute<T>(stream, L, W, filterList, nFilter, S, (T*)ftmp, (T*)rhoPre, (T*)rhoEPre);
     launchFilterWrite<T>(stream, L, filterList, nFilter, S, (const T*)ftmp);
@@ -470,7 +484,8 @@ void Case::stepOnce() {
             nStage4Partials = launchStageFused<T>(stream, st, L, P, W, in[st - 1], u, out[st - 1], a, cls,
                                                   (const T*)rhoPre, (const T*)rhoEPre, dtdev, tile,
                                                   fusedScans ? partials : nullptr, cfg.conv_history > 0,
-                                                  rhoFilter ? fRho : nullptr, rhoFilter ? fFlag : nullptr, comp);
+                                                  rhoFilter ? fRho : nullptr, rhoFilter ? fFlag : nullptr, comp,
+                                                  stageOutlet());
             mark(0, false);
         } else {
             mark(1, true);
@@ -483,7 +498,7 @@ void Case::stepOnce() {
         }
         if (st == 1 && rhoFilter) launchRhoCommit<T>(stream, L, u, fRho, a, cls, fFlag);
         mark(2, true);
-        applyBCs<T>(out[st - 1], frac[st - 1]);
+        applyBCs<T>(out[st - 1], frac[st - 1], stageKernel == 1 && stageOutlet());
         mark(2, false);
         if (st < 4) tapeCopy(st, out[st - 1]);
     }
@@ -537,12 +552,18 @@ void Case::setOption(const std::string& name, int64_t value) {
         tile.kc = (int)value;
     } else if (name == "global_access") {
         tile.globalAccess = value != 0;
+    } else if (name == "fused_outlet") {
+        fuseOutletOpt = value != 0;
     } else if (name == "autotune") {
         if (!initialized) throw StateError("autotune before init");
         autotune(value > 0 ? (int)value : 3);
     } else {
         throw std::invalid_argument("unknown option " + name);
     }
+}
+
+const EdgeFace* Case::stageOutlet() const {
+    return fuseOutletOpt && fusedOutlet >= 0 ? &faces[fusedOutlet] : nullptr;
 }
 
 double Case::autotune(int reps) {
@@ -569,11 +590,11 @@ double Case::autotune(int reps) {
                     if (fp32)
                         launchStageFused<float>(stream, 2, L, P, W, (const float*)U, (const float*)U, (float*)S2, (float*)A,
                                                 cls, (const float*)rhoPre, (const float*)rhoEPre, &scal->dt, tl,
-                                                nullptr, 0, nullptr, nullptr, comp);
+                                                nullptr, 0, nullptr, nullptr, comp, stageOutlet());
                     else
                         launchStageFused<double>(stream, 2, L, P, W, (const double*)U, (const double*)U, (double*)S2,
                                                  (double*)A, cls, (const double*)rhoPre, (const double*)rhoEPre,
-                                                 &scal->dt, tl, nullptr, 0, nullptr, nullptr, nullptr);
+                                                 &scal->dt, tl, nullptr, 0, nullptr, nullptr, nullptr, stageOutlet());
                 }
             };
             run(1);  // warm-up

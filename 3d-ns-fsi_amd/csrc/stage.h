@@ -68,11 +68,21 @@ constexpr int kMinTileKc = 8;  // smallest z-chunk accepted (sizes the scan part
 // comp != null (FP32 only; both stage kernels): compensated combine. The accumulator A holds the step's
 // increment at interior nodes (stages 1-3) and stage 4 forms U + increment with the carried rounding
 // error; comp is 5 bfloat16 fields with the layout's element offsets, zero before the first step.
+// outlet != null: that outlet face (outletFusable) is applied to the stage output (Sout, A in stage 4) by
+// the kernel itself, with the values the edge-face pass would give it when run first after the kernel.
 template <class T>
 int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
                      const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
-                     const int32_t* filterFlag, uint16_t* comp);
+                     const int32_t* filterFlag, uint16_t* comp, const EdgeFace* outlet = nullptr);
+// Whether the fused stage kernel can apply edge face f itself: an outlet at x = NI-1 whose row's last
+// x-segment holds at least three nodes (the node and the two it extrapolates from share a wavefront).
+// The caller also keeps f out of the edge-face pass and checks that f is independent of every face
+// applied before it.
+inline bool outletFusable(const Layout& L, const EdgeFace& f) {
+    const int tail = L.NI - Layout::kSegNodes * ((L.NI + Layout::kSegNodes - 1) / Layout::kSegNodes - 1);
+    return f.type == NS_BC_OUTLET && f.axis == 0 && f.side == 1 && f.i0 == L.NI - 1 && f.i1 == L.NI - 1 && tail >= 3;
+}
 const int* fusedTileHeights(int* n);
 const int* fusedChunkPlanes(int* n);  // z-chunk lengths the autotuners try
 // Batched FP32 fused stage: fusedArgsBytes() bytes of arguments per case (buildFusedArgs fills them and
@@ -81,7 +91,8 @@ const int* fusedChunkPlanes(int* n);  // z-chunk lengths the autotuners try
 size_t fusedArgsBytes();
 int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const Window& W, const float* Sin,
                    const float* U, float* Sout, float* A, const uint8_t* cls, const float* rhoPre, const float* rhoEPre,
-                   const double* dtdev, FusedTile tile, double* partials, int norms, uint16_t* comp);
+                   const double* dtdev, FusedTile tile, double* partials, int norms, uint16_t* comp,
+                   const EdgeFace* outlet = nullptr);
 void launchStageBatched(hipStream_t s, int stage, int ty, bool scan, bool kahan, const void* tab, const int* start,
                         int ncases, int total);
 // Force on the body from the state S after a step's boundary application (FP64; the discrete momentum

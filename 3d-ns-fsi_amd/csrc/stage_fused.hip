@@ -91,6 +91,8 @@ struct FusedArgs {
     int nbx, nby, nbz, kc;   // tiles in x, y, z and planes per z-chunk
     int tailHalf;            // the last x-segment holds at most 30 nodes: its tiles are half-width
     int nblocks;
+    // outlet face at x = NI-1 applied here (its owned rows j0..j1 x k0..k1), instead of by the edge-face pass
+    int fuseOutlet, oj0, oj1, ok0, ok1;
     // stage 4 only: per-block partials of the end-of-step scans over interior, non-filter nodes
     // (slots: 3 spectral-radius maxima, viscous maximum, 5 sums of squared change, non-finite count)
     double* part;
@@ -641,6 +643,21 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                     }
                 }
             }
+            if (a.fuseOutlet && bx == a.nbx - 1) {
+                // the outlet node (NI-1, j, k) takes the outlet values from this stage's results at NI-2 and
+                // NI-3, two and one lanes down in the same (half-)tile (the row's last segment holds at least
+                // three nodes), exactly as the edge-face pass right after this kernel would
+                T* v = STAGE == 4 ? vA : vS;
+                T n1[4], n2[4], ov[5];
+                for (int q = 0; q < 4; ++q) {
+                    n1[q] = shr1(v[q]);
+                    n2[q] = shr1(n1[q]);
+                }
+                if (i == L.NI - 1 && j >= a.oj0 && j <= a.oj1 && k >= a.ok0 && k <= a.ok1) {
+                    outletValues(c.gm1, n1, n2, ov);
+                    for (int q = 0; q < 5; ++q) v[q] = ov[q];
+                }
+            }
             if (STAGE != 4 && wS)
                 for (int q = 0; q < 5; ++q) m.stSout(q, bo, vS[q]);
             if (wA)
@@ -897,7 +914,8 @@ int normTY(int ty) { return ty == 4 || ty == 6 || ty == 14 ? ty : 8; }
 template <class T>
 void fillArgs(FusedArgs<T>& a, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin, const T* U,
               T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre, const double* dtdev, FusedTile tile,
-              double* partials, int norms, const T* filteredRho, const int32_t* filterFlag, uint16_t* comp) {
+              double* partials, int norms, const T* filteredRho, const int32_t* filterFlag, uint16_t* comp,
+              const EdgeFace* outlet) {
     a = FusedArgs<T>{};
     if (comp && sizeof(T) != 4) throw std::invalid_argument("compensated combine needs FP32 fields");
     a.fRho = stage == 1 ? filteredRho : nullptr;
@@ -927,6 +945,14 @@ void fillArgs(FusedArgs<T>& a, int stage, const Layout& L, const Params& P, cons
     a.nbz = (L.NK + a.kc - 1) / a.kc;
     a.tailHalf = L.NI - Layout::kSegNodes * (a.nbx - 1) <= 30;
     a.nblocks = ((a.nbx - a.tailHalf) * a.nby + (a.tailHalf ? (a.nby + 1) / 2 : 0)) * a.nbz;
+    if (outlet) {
+        if (!outletFusable(L, *outlet)) throw std::invalid_argument("outlet face cannot be applied in the stage kernel");
+        a.fuseOutlet = 1;
+        a.oj0 = outlet->j0;
+        a.oj1 = outlet->j1;
+        a.ok0 = outlet->k0;
+        a.ok1 = outlet->k1;
+    }
 }
 
 }  // namespace
@@ -935,10 +961,10 @@ template <class T>
 int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
                      const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
-                     const int32_t* filterFlag, uint16_t* comp) {
+                     const int32_t* filterFlag, uint16_t* comp, const EdgeFace* outlet) {
     FusedArgs<T> a;
     fillArgs(a, stage, L, P, W, Sin, U, Sout, A, cls, rhoPre, rhoEPre, dtdev, tile, partials, norms, filteredRho,
-             filterFlag, comp);
+             filterFlag, comp, outlet);
     const bool buf = !tile.globalAccess && stateFitsBuffer(L);
     switch (normTY(tile.ty)) {
         case 4: launchTY<T, 4>(s, stage, a, buf); break;
@@ -953,11 +979,12 @@ size_t fusedArgsBytes() { return sizeof(FusedArgs<float>); }
 
 int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const Window& W, const float* Sin,
                    const float* U, float* Sout, float* A, const uint8_t* cls, const float* rhoPre, const float* rhoEPre,
-                   const double* dtdev, FusedTile tile, double* partials, int norms, uint16_t* comp) {
+                   const double* dtdev, FusedTile tile, double* partials, int norms, uint16_t* comp,
+                   const EdgeFace* outlet) {
     if (!stateFitsBuffer(L)) throw std::length_error("batched case: state buffer beyond 4 GiB");
     FusedArgs<float> a;
     fillArgs<float>(a, stage, L, P, W, Sin, U, Sout, A, cls, rhoPre, rhoEPre, dtdev, tile, partials, norms, nullptr,
-                    nullptr, comp);
+                    nullptr, comp, outlet);
     std::memcpy(dst, &a, sizeof a);
     return a.nblocks;
 }
@@ -1022,10 +1049,10 @@ const int* fusedChunkPlanes(int* n) {
 template int launchStageFused<double>(hipStream_t, int, const Layout&, const Params&, const Window&, const double*,
                                       const double*, double*, double*, const uint8_t*, const double*, const double*,
                                       const double*, FusedTile, double*, int, const double*, const int32_t*,
-                                      uint16_t*);
+                                      uint16_t*, const EdgeFace*);
 template int launchStageFused<float>(hipStream_t, int, const Layout&, const Params&, const Window&, const float*,
                                      const float*, float*, float*, const uint8_t*, const float*, const float*,
                                      const double*, FusedTile, double*, int, const float*, const int32_t*,
-                                     uint16_t*);
+                                     uint16_t*, const EdgeFace*);
 
 }  // namespace nsfsi

@@ -226,6 +226,10 @@ ns_status ns_profile(ns_handle* h, int32_t nsteps, double ms[4], int32_t launche
  *   "global_access" FP32 fused stage: 1 addresses the fields by per-field global pointers (the form
  *                  used for states beyond 4 GiB, which one buffer resource cannot span), 0 (default)
  *                  by buffer resources where the state fits. Same results either way.
+ *   "fused_outlet" 1 (default): an outlet face at x = NI-1 is applied by the fused stage kernel itself
+ *                  when a row's last x-segment (62 nodes per segment) holds at least three nodes and the
+ *                  face is independent of the faces applied before it; 0 leaves it to the edge-face pass.
+ *                  Same results either way.
  * Results do not depend on the tile shape (the per-node arithmetic is the same). */
 ns_status ns_set_option(ns_handle* h, const char* name, int64_t value);
 
