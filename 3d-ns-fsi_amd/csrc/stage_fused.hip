@@ -18,7 +18,8 @@
  * y-halo. A row's last segment, when it holds at most 30 nodes, is tiled at half width: each half of the
  * wave takes it in one of two y-bands, so a 512-node row costs 8.5 waves instead of 9 and a 64-node row
  * 1.5 instead of 2. The block streams through a chunk of z-planes: the own column's neighbours in z live in
- * registers, x-neighbours come by cross-lane shuffle, y-neighbours through LDS. Primitive variables are
+ * registers, x-neighbours come by cross-lane shuffle, y-neighbours through LDS (two buffers in FP32, so a
+ * plane needs one barrier). Primitive variables are
  * recomputed from the conserved state; at immersed-filter nodes from the pre-filter density and energy
  * (the reference keeps primitives pre-filter there). Each output wave stores its whole segment, padding
  * slots included, with streaming stores, so every line it writes is written whole. Blocks are mapped so
@@ -335,10 +336,11 @@ __device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, const M& m, int i
     return r;
 }
 
-// Primitives and in-plane derivatives of a fetched plane; x-derivatives by shuffle, y through LDS.
+// Primitives and x-derivatives (by shuffle) of a fetched plane; the velocities are published in LDS for
+// the y-derivatives, which planeDy reads after a barrier.
 template <class T, int TY>
-__device__ __forceinline__ void finishPlane(const FusedArgs<T>& a, Smem<T, TY>& sm, int row, int lane, const Raw<T>& r,
-                                            Pt<T>& q, bool needDy) {
+__device__ __forceinline__ void planePrims(const FusedArgs<T>& a, Smem<T, TY>& sm, int row, int lane, const Raw<T>& r,
+                                           Pt<T>& q) {
     q.rho = r.rho;
     q.ru = r.ru;
     q.rv = r.rv;
@@ -352,8 +354,13 @@ __device__ __forceinline__ void finishPlane(const FusedArgs<T>& a, Smem<T, TY>& 
     sm.uvw[0][row][lane] = q.u;
     sm.uvw[1][row][lane] = q.v;
     sm.uvw[2][row][lane] = q.w;
-    __syncthreads();
-    // halo rows never use y-derivatives (they only supply G_y, which needs x- and z-derivatives)
+}
+
+// y-derivatives from the published velocities. Halo rows never use them (they only supply G_y, which
+// needs x- and z-derivatives).
+template <class T, int TY>
+__device__ __forceinline__ void planeDy(const FusedArgs<T>& a, const Smem<T, TY>& sm, int row, int lane, Pt<T>& q,
+                                        bool needDy) {
     if (needDy) {
         q.dyu = (sm.uvw[0][row + 1][lane] - sm.uvw[0][row - 1][lane]) * a.c.i2dy;
         q.dyv = (sm.uvw[1][row + 1][lane] - sm.uvw[1][row - 1][lane]) * a.c.i2dy;
@@ -362,6 +369,19 @@ __device__ __forceinline__ void finishPlane(const FusedArgs<T>& a, Smem<T, TY>& 
         q.dyu = q.dyv = q.dyw = 0;
     }
 }
+
+template <class T, int TY>
+__device__ __forceinline__ void finishPlane(const FusedArgs<T>& a, Smem<T, TY>& sm, int row, int lane, const Raw<T>& r,
+                                            Pt<T>& q, bool needDy) {
+    planePrims<T, TY>(a, sm, row, lane, r, q);
+    __syncthreads();
+    planeDy<T, TY>(a, sm, row, lane, q, needDy);
+}
+
+// LDS buffers of the z-loop: two in FP32 (planes alternate, one barrier per plane; two blocks of 6 output
+// rows still fit a CU), one in FP64 (a second barrier per plane guards its reuse).
+template <class T>
+constexpr int smemBuffers() { return sizeof(T) == 4 ? 2 : 1; }
 
 template <class T, int TY>
 __device__ __forceinline__ void readFaceNb(const Smem<T, TY>& sm, int r, int lane, Pt<T>& nb, T g[5]) {
@@ -373,7 +393,7 @@ __device__ __forceinline__ void readFaceNb(const Smem<T, TY>& sm, int r, int lan
     nb.Tt = sm.face[9][r][lane];
 }
 
-// Tile lb of a z-chunk order: (x-segment, y-band, z-chunk) and whether the tile is half-width.
+// Position of a tile: x-segment, y-band (the first of two for a half-width tile), z-chunk.
 struct TileIdx {
     int bx, by, bz;
     bool half;
@@ -412,8 +432,9 @@ __device__ __forceinline__ TileIdx tileOf(int lb, int nbx, int nby, bool tailHal
 // segment in y-band by, lanes 32..63 in band by+1, each half with its own halo lanes (sub-lane 0 and the
 // sub-lanes past the last node), so a wave shift across the halves only moves values between halo lanes.
 template <class T, int STAGE, int TY, bool SCAN, bool KAHAN, bool BUF>
-__device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, Smem<T, TY>& sm,
+__device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, Smem<T, TY> (&smb)[smemBuffers<T>()],
                                           double (&wred)[TY + 2][10]) {
+    constexpr int NB = smemBuffers<T>();
     const TileIdx tl = tileOf(lb, a.nbx, a.nby, a.tailHalf);
     const int bx = tl.bx, bz = tl.bz;
     const int lane = threadIdx.x, row = threadIdx.y;
@@ -455,10 +476,10 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     Pt<T> qm, q0, q1;
     Raw<T> rnext = fetch(a, m, ic, jc, colc, kb - 1);
     const bool inner = row >= 1 && row <= TY;  // output rows (halo rows are 0 and TY+1)
-    finishPlane<T, TY>(a, sm, row, lane, rnext, qm, inner);
+    finishPlane<T, TY>(a, smb[0], row, lane, rnext, qm, inner);
     rnext = fetch(a, m, ic, jc, colc, kb);
-    __syncthreads();
-    finishPlane<T, TY>(a, sm, row, lane, rnext, q0, inner);
+    if (NB == 1) __syncthreads();  // the readers of plane kb-1 are done before its buffer is rewritten
+    finishPlane<T, TY>(a, smb[NB - 1], row, lane, rnext, q0, inner);
     rnext = fetch(a, m, ic, jc, colc, kb + 1);
     T gz0[5], phizm[5];
     if (inner) {
@@ -475,9 +496,15 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     // block partials are combined in FP64
     T red[10];
     for (int q = 0; q < 10; ++q) red[q] = 0;
-    __syncthreads();  // the prologue's readers of sm.uvw are done before the loop rewrites it
+    // One buffer: the prologue's readers of uvw are done before the loop rewrites it. Two buffers: plane k
+    // uses buffer (k - kb) & 1, and whoever writes it again (plane k + 2) has passed the barrier of plane
+    // k + 1, which every wave reaches after its reads of plane k; the prologue's planes use buffer 0
+    // (plane kb-1, read before the barrier of plane kb) and buffer 1 (plane kb, read before the barrier of
+    // the loop's first plane).
+    if (NB == 1) __syncthreads();
 
     for (int k = kb; k < ke; ++k) {
+        Smem<T, TY>& sm = smb[NB == 2 ? (k - kb) & 1 : 0];
         const Raw<T> rcur = rnext;
         rnext = fetch(a, m, ic, jc, colc, k + 2);
         // combine inputs of the output node, fetched before the stencil work
@@ -505,8 +532,21 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                 for (int q = 0; q < 5; ++q) acc[q] = m.ldA(q, bo);
             }
         }
-        // sm.uvw is rewritten here: every reader of the previous plane's uvw has passed the face barrier
-        finishPlane<T, TY>(a, sm, row, lane, rcur, q1, inner);
+        // plane k+1: primitives and x-derivatives, velocities published for its y-derivatives
+        planePrims<T, TY>(a, sm, row, lane, rcur, q1);
+        const T dzu = (q1.u - umz) * c.i2dz, dzv = (q1.v - vmz) * c.i2dz, dzw = (q1.w - wmz) * c.i2dz;
+        // y-faces of plane k: every row publishes G_y and its compact inputs; output rows evaluate both
+        // faces (G_y needs no y-derivative, so it goes out with plane k+1's velocities under one barrier)
+        T g[5];
+        gy(c, q0, dzv, dzw, g);
+        for (int q = 0; q < 5; ++q) sm.face[q][row][lane] = g[q];
+        sm.face[5][row][lane] = q0.mu;
+        sm.face[6][row][lane] = q0.u;
+        sm.face[7][row][lane] = q0.v;
+        sm.face[8][row][lane] = q0.w;
+        sm.face[9][row][lane] = q0.Tt;
+        __syncthreads();
+        planeDy<T, TY>(a, sm, row, lane, q1, inner);
         // z-face above plane k (output rows only)
         T gz1[5], phiz[5];
         if (inner) {
@@ -517,17 +557,6 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
         } else {
             for (int q = 0; q < 5; ++q) gz1[q] = phiz[q] = 0;
         }
-        const T dzu = (q1.u - umz) * c.i2dz, dzv = (q1.v - vmz) * c.i2dz, dzw = (q1.w - wmz) * c.i2dz;
-        // y-faces: every row publishes G_y and its compact inputs; output rows evaluate both faces
-        T g[5];
-        gy(c, q0, dzv, dzw, g);
-        for (int q = 0; q < 5; ++q) sm.face[q][row][lane] = g[q];
-        sm.face[5][row][lane] = q0.mu;
-        sm.face[6][row][lane] = q0.u;
-        sm.face[7][row][lane] = q0.v;
-        sm.face[8][row][lane] = q0.w;
-        sm.face[9][row][lane] = q0.Tt;
-        __syncthreads();
         if (outRow) {
             T rhs[5];
             {
@@ -676,6 +705,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
             gz0[q] = gz1[q];
             phizm[q] = phiz[q];
         }
+        if (NB == 1) __syncthreads();  // one buffer: this plane's readers are done before it is rewritten
     }
     if (SCAN) {
         // block partials: wave reduction, then one value per wave through LDS
@@ -698,7 +728,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
 template <class T, int STAGE, int TY, bool SCAN, bool KAHAN, bool BUF>
 __global__ __launch_bounds__(64 * (TY + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void k_stage_fused(FusedArgs<T> a) {
-    __shared__ Smem<T, TY> sm;
+    __shared__ Smem<T, TY> sm[smemBuffers<T>()];
     __shared__ double wred[TY + 2][10];
     // XCD-aware mapping: hardware block p runs on XCD p % 8; give each XCD a contiguous range of tiles
     const int p = blockIdx.x;
@@ -714,7 +744,7 @@ void k_stage_fused(FusedArgs<T> a) {
 template <class T, int STAGE, int TY, bool SCAN, bool KAHAN>
 __global__ __launch_bounds__(64 * (TY + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void k_stage_batched(const FusedArgs<T>* __restrict__ tab, const int* __restrict__ start, int ncases, int total) {
-    __shared__ Smem<T, TY> sm;
+    __shared__ Smem<T, TY> sm[smemBuffers<T>()];
     __shared__ double wred[TY + 2][10];
     const int p = blockIdx.x;
     const int per = (total + 7) / 8;
