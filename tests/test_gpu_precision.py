@@ -67,6 +67,29 @@ def test_fp32_error_vs_oracle(sphere48_200):
         assert np.all(l2 < 1e-4)
 
 
+@pytest.fixture(scope="module")
+def sphere48_200_floor():
+    """The oracle's own 1-ulp sensitivity on the 200-step 48^3 sphere (see test_gpu_parity.ulp_sensitivity)."""
+    from test_gpu_parity import ulp_sensitivity
+    return np.array(ulp_sensitivity(cases.sphere48(N=48, steps=200), 200))
+
+
+@pytest.mark.parametrize("kernel", [1, 0], ids=["fused", "unfused"])
+def test_fp64_matches_oracle_200_steps(sphere48_200, sphere48_200_floor, kernel):
+    """The 48^3 sphere for 200 steps (the reference's CPU-runnable configuration) in FP64 on the device:
+    same time level and time as the oracle, and per conserved field within 1e-12 relative L2 of it or
+    within 4x the oracle's own change under a 1-ulp perturbation, whichever is larger (the transverse
+    momenta, whose norms are three orders below the axial one, sit at that floor: about 2e-12)."""
+    cfg = cases.sphere48(N=48, steps=200, precision="fp64")
+    U, (t, lv, dt) = run_device(cfg, 200, kernel)
+    assert lv == 200
+    assert abs(t - sphere48_200["t"]) <= 1e-12 * sphere48_200["t"]
+    l2, linf = errors(U, sphere48_200["ref"])
+    floor = sphere48_200_floor
+    print(f"FP64 kernel={kernel} rel L2 {l2} rel Linf {linf} oracle 1-ulp sensitivity {floor}")
+    assert np.all(l2 <= np.maximum(1e-12, 4 * floor)), (l2, floor)
+
+
 def test_kahan_not_less_accurate(sphere48_200):
     ref = sphere48_200["ref"]
     k_l2, _ = errors(sphere48_200[(1, 1)][0], ref)
