@@ -481,6 +481,11 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     if (NB == 1) __syncthreads();  // the readers of plane kb-1 are done before its buffer is rewritten
     finishPlane<T, TY>(a, smb[NB - 1], row, lane, rnext, q0, inner);
     rnext = fetch(a, m, ic, jc, colc, kb + 1);
+    // Stages 1-3 fetch the stencil input two planes ahead of its use, stage 4 (whose registers go to the
+    // scans and the compensation) one plane ahead: two planes there spill.
+    constexpr bool kDeep = STAGE != 4;
+    Raw<T> rnext2{};
+    if constexpr (kDeep) rnext2 = fetch(a, m, ic, jc, colc, kb + 2);
     T gz0[5], phizm[5];
     if (inner) {
         T gzm[5], cz[5];
@@ -506,7 +511,12 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     for (int k = kb; k < ke; ++k) {
         Smem<T, TY>& sm = smb[NB == 2 ? (k - kb) & 1 : 0];
         const Raw<T> rcur = rnext;
-        rnext = fetch(a, m, ic, jc, colc, k + 2);
+        if constexpr (kDeep) {
+            rnext = rnext2;
+            rnext2 = fetch(a, m, ic, jc, colc, k + 3);
+        } else {
+            rnext = fetch(a, m, ic, jc, colc, k + 2);
+        }
         // combine inputs of the output node, fetched before the stencil work
         const bool doOut = outRow && outLane;
         const uint32_t bo = outRow ? colo + (uint32_t)k * planeB : 0;
