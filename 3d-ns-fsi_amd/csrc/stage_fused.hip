@@ -481,9 +481,9 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     if (NB == 1) __syncthreads();  // the readers of plane kb-1 are done before its buffer is rewritten
     finishPlane<T, TY>(a, smb[NB - 1], row, lane, rnext, q0, inner);
     rnext = fetch(a, m, ic, jc, colc, kb + 1);
-    // Stages 1-3 fetch the stencil input two planes ahead of its use, stage 4 (whose registers go to the
-    // scans and the compensation) one plane ahead: two planes there spill.
-    constexpr bool kDeep = STAGE != 4;
+    // The stencil input is fetched two planes ahead of its use, except in the FP32 stage 4, whose
+    // registers go to the scans and the compensation (two planes ahead spill there).
+    constexpr bool kDeep = STAGE != 4 || sizeof(T) == 8;
     Raw<T> rnext2{};
     if constexpr (kDeep) rnext2 = fetch(a, m, ic, jc, colc, kb + 2);
     T gz0[5], phizm[5];
