@@ -72,6 +72,21 @@ CLASSIFY_CASES = {
         body_centre=(2.485, 0.61, 0.59), body_radius=0.3, stop_criterion="timesteps", stop_time_level=5,
         filter_intervals=[0], filter_change_times=[], ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=11,
         M_0=0.3, Re=300.0, conv_stability_limit=0.5),
+    # inflow along y (inlet y-min, outlet y-max, symmetry on x and z): the edge faces and their claims on
+    # another axis, with the freestream initial field still along x
+    "sphere_inflow_y": lambda: nsfsi.make_config(
+        NI=30, NJ=34, NK=26, L_x=2.2, L_y=2.6, L_z=2.0, bc=["symmetry", "symmetry", "inlet", "outlet", "symmetry",
+                                                         "symmetry"],
+        body_type="sphere", body_centre=(1.08, 1.21, 0.97), body_radius=0.45, stop_criterion="timesteps",
+        stop_time_level=5, filter_intervals=[0], filter_change_times=[], ic_type="freestream_gaussian",
+        ic_amplitude=1e-3, ic_seed=13, M_0=0.3, Re=250.0, conv_stability_limit=0.5),
+    # periodic x, inflow along z, an x-aligned cylinder
+    "cylinder_x_inflow_z": lambda: nsfsi.make_config(
+        NI=28, NJ=26, NK=36, L_x=2.0, L_y=2.0, L_z=2.8, bc=["periodic", "periodic", "symmetry", "symmetry", "inlet",
+                                                         "outlet"],
+        body_type="cylinder", body_axis="x", body_centre=(1.0, 0.98, 1.31), body_radius=0.4, stop_criterion="timesteps",
+        stop_time_level=5, filter_intervals=[0], filter_change_times=[], ic_type="freestream_gaussian",
+        ic_amplitude=1e-3, ic_seed=17, M_0=0.25, Re=200.0, conv_stability_limit=0.5),
 }
 
 
@@ -96,7 +111,8 @@ def test_classification_bit_exact(name):
     assert np.array_equal(s.mesh.positionOffset, of)
 
 
-@pytest.mark.parametrize("name", ["sphere24", "sphere48", "cylinder_z64", "sphere_offcentre33", "sphere_seam100"])
+@pytest.mark.parametrize("name", ["sphere24", "sphere48", "cylinder_z64", "sphere_offcentre33", "sphere_seam100",
+                                  "sphere_inflow_y", "cylinder_x_inflow_z"])
 def test_initial_state_bit_exact(name):
     cfg = CLASSIFY_CASES[name]()
     s, o = _pair(cfg)
@@ -106,7 +122,8 @@ def test_initial_state_bit_exact(name):
 
 @pytest.mark.parametrize("kernel", [1, 0], ids=["fused", "unfused"])
 @pytest.mark.parametrize("name,steps", [("sphere24", 20), ("sphere48", 20), ("cylinder_z64", 10),
-                                        ("sphere_offcentre33", 15), ("sphere_seam100", 15)])
+                                        ("sphere_offcentre33", 15), ("sphere_seam100", 15), ("sphere_inflow_y", 15),
+                                        ("cylinder_x_inflow_z", 15)])
 def test_trajectory_fp64(name, steps, kernel):
     cfg = CLASSIFY_CASES[name]()
     s, o = _pair(cfg)
