@@ -19,7 +19,23 @@ def small_sweep(n=6, steps=8):
     cyl = cases.cylinder256(N=40, steps=steps, precision="fp32")
     cyl.NK = 20
     cyl.bc[4] = cyl.bc[5] = nsfsi.BC["periodic"]
-    return cfgs + [cyl]
+    # inflow along y, and along z with periodic x: faces of other axes in the batched face groups, and
+    # outlets that the stage kernel leaves to the edge-face pass beside x-outlets that it applies itself
+    yin = nsfsi.make_config(
+        NI=30, NJ=34, NK=26, L_x=2.2, L_y=2.6, L_z=2.0, bc=["symmetry", "symmetry", "inlet", "outlet", "symmetry",
+                                                         "symmetry"],
+        body_type="sphere", body_centre=(1.08, 1.21, 0.97), body_radius=0.45, stop_criterion="timesteps",
+        stop_time_level=steps, filter_intervals=[0], filter_change_times=[], ic_type="freestream_gaussian",
+        ic_amplitude=1e-3, ic_seed=13, M_0=0.3, Re=250.0, conv_stability_limit=0.5, precision="fp32")
+    zin = nsfsi.make_config(
+        NI=28, NJ=26, NK=36, L_x=2.0, L_y=2.0, L_z=2.8, bc=["periodic", "periodic", "symmetry", "symmetry", "inlet",
+                                                         "outlet"],
+        body_type="cylinder", body_axis="x", body_centre=(1.0, 0.98, 1.31), body_radius=0.4, stop_criterion="timesteps",
+        stop_time_level=steps, filter_intervals=[0], filter_change_times=[], ic_type="freestream_gaussian",
+        ic_amplitude=1e-3, ic_seed=17, M_0=0.25, Re=200.0, conv_stability_limit=0.5, precision="fp32")
+    for c in (yin, zin):
+        c.kahan = cfgs[0].kahan
+    return cfgs + [cyl, yin, zin]
 
 
 def single(cfg, steps, tile):
