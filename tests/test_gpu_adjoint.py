@@ -37,10 +37,10 @@ def J_at(cfg, U, q, re, nsteps, n0, n1, dts):
     return nt.drag_and_gradients(cfg, U, q, re, nsteps, n0, n1, checkpoint=nsteps, dts=dts)["J"]
 
 
-def check_u0(nsteps, n0, n1, ckpt, ndir=3, N=24):
-    cfg = small_case(N)
+def check_u0(nsteps, n0, n1, ckpt, ndir=3, N=24, cfg=None):
+    cfg = cfg if cfg is not None else small_case(N)
     U, q = base(cfg)
-    print(f"adjoint N={N} nsteps={nsteps} window={n0}..{n1} checkpoint={ckpt}", flush=True)
+    print(f"adjoint grid={cfg.NI}x{cfg.NJ}x{cfg.NK} nsteps={nsteps} window={n0}..{n1} checkpoint={ckpt}", flush=True)
     r = nt.drag_and_gradients(cfg, U, q, cfg.Re, nsteps, n0, n1, ckpt)
     assert np.all(np.isfinite(r["grad_u0"]))
     rng = np.random.default_rng(nsteps * 100 + n0)
@@ -69,6 +69,20 @@ def test_one_step():
 
 def test_several_steps_two_segments():
     check_u0(7, 3, 7, 4)
+
+
+def test_other_face_axes():
+    """Inflow along y (inlet y-min, outlet y-max, symmetry on x and z) and inflow along z with periodic x:
+    the transposed edge faces of the other axes and the periodic copy."""
+    for bc, dims in ((["symmetry", "symmetry", "inlet", "outlet", "symmetry", "symmetry"], (20, 24, 18)),
+                     (["periodic", "periodic", "symmetry", "symmetry", "inlet", "outlet"], (18, 20, 24))):
+        ni, nj, nk = dims
+        cfg = nsfsi.make_config(
+            NI=ni, NJ=nj, NK=nk, L_x=2.0, L_y=2.2, L_z=2.1, bc=bc, body_type="sphere", body_centre=(1.0, 1.1, 1.05),
+            body_radius=0.45, stop_criterion="timesteps", stop_time_level=6, filter_intervals=[0],
+            filter_change_times=[], ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=19, M_0=0.3, Re=200.0,
+            conv_stability_limit=0.5, inlet_ramp_time=0.02)
+        check_u0(6, 2, 6, 3, cfg=cfg)
 
 
 def test_inflow_gradient():
