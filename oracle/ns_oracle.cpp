@@ -1026,6 +1026,48 @@ const char* ns_oracle_last_error(void* h) { return h ? ((Oracle*)h)->err.c_str()
 
 void ns_oracle_destroy(void* h) { delete (Oracle*)h; }
 
+/* Unit hook: set t and timeLevel (the periodic filter's schedule reads both, src/Mesh.cpp:85-104). */
+int ns_oracle_set_time(void* h, double t, uint64_t level) {
+    auto* o = (Oracle*)h;
+    if (!o) return NS_ERR_ARGUMENT;
+    o->t = t;
+    o->timeLevel = level;
+    return NS_OK;
+}
+
+/* Unit hook: Mesh::checkFilterCondition (src/Mesh.cpp:85-104) at the current t and timeLevel. */
+int ns_oracle_filter_now(void* h) {
+    auto* o = (Oracle*)h;
+    return o ? (o->filterNow() ? 1 : 0) : -1;
+}
+
+/* Unit hook: Mesh::applyFilter_ifAppropriate (src/Mesh.cpp:107-140) on the current density, as the
+ * step does it before the buffer swap; returns 1 when the schedule fired. */
+int ns_oracle_rho_filter(void* h) {
+    auto* o = (Oracle*)h;
+    if (!o) return -1;
+    bool f = o->filterNow();
+    o->applyRhoFilter();
+    return f ? 1 : 0;
+}
+
+/* Unit hook: Mesh::getNormOfChange (src/Mesh.cpp:171-178) of variable q between the "Old" and the
+ * current conserved arrays. */
+double ns_oracle_norm_of_change(void* h, int q) {
+    auto* o = (Oracle*)h;
+    if (!o || q < 0 || q > 4) return -1;
+    return o->normOfChange(q);
+}
+
+/* Unit hook: Solver::updateTimeStepSize (src/Solver.cpp:80-127) at tnow from the current arrays,
+ * without deriving anything first; returns the new dt. */
+double ns_oracle_update_dt(void* h, double tnow) {
+    auto* o = (Oracle*)h;
+    if (!o) return -1;
+    o->updateTimeStepSize(tnow);
+    return o->dt;
+}
+
 /* Unit hook: 8x8 full-pivot LU solve of A x = b (A row-major). */
 void ns_oracle_lu8_solve(const double* A, const double* b, double* x) {
     double LU[64];

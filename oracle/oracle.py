@@ -62,6 +62,13 @@ def lib():
     L.ns_oracle_lu8_solve.restype = None
     L.ns_oracle_uniform01.argtypes = [C.c_uint64, C.c_uint64]
     L.ns_oracle_uniform01.restype = C.c_double
+    L.ns_oracle_set_time.argtypes = [H, C.c_double, C.c_uint64]
+    L.ns_oracle_filter_now.argtypes = [H]
+    L.ns_oracle_rho_filter.argtypes = [H]
+    L.ns_oracle_norm_of_change.argtypes = [H, C.c_int]
+    L.ns_oracle_norm_of_change.restype = C.c_double
+    L.ns_oracle_update_dt.argtypes = [H, C.c_double]
+    L.ns_oracle_update_dt.restype = C.c_double
     _lib = L
     return L
 
@@ -170,6 +177,29 @@ class Oracle:
         if n > 0:
             lib().ns_oracle_norm_history(self._h, int(var), _d(out), n)
         return out
+
+    def set_old_field(self, q, values):
+        """Overwrite the "Old" copy of conserved variable q (the buffer the norm of change reads)."""
+        self.set_field(12 + int(q), values)
+
+    def old_field(self, q) -> np.ndarray:
+        return self.field(12 + int(q))
+
+    def set_time(self, t, level):
+        self._check(lib().ns_oracle_set_time(self._h, float(t), int(level)))
+
+    def filter_now(self) -> bool:
+        return lib().ns_oracle_filter_now(self._h) == 1
+
+    def rho_filter(self) -> bool:
+        """The periodic density filter as the step applies it; True when its schedule fired."""
+        return lib().ns_oracle_rho_filter(self._h) == 1
+
+    def norm_of_change(self, q) -> float:
+        return lib().ns_oracle_norm_of_change(self._h, int(q))
+
+    def update_dt(self, tnow) -> float:
+        return lib().ns_oracle_update_dt(self._h, float(tnow))
 
     def close(self):
         if self._h:
