@@ -41,6 +41,7 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
     if (nGhost == 0) throw Unsupported("the drag objective needs an immersed body");
     if (nsteps < 1 || n0 < 0 || n1 < n0 || n1 > nsteps || ckpt < 1)
         throw std::invalid_argument("need 1 <= nsteps, 0 <= n0 <= n1 <= nsteps, ckpt >= 1");
+    checkSchedule(nsteps);  // a prescribed schedule must cover the forward run
     dropGraphs();
     const bool graphWas = useGraph, forceWas = bodyForce;
     useGraph = false;

@@ -12,6 +12,7 @@
 #include "batch_host.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 
 namespace nsfsi {
@@ -35,9 +36,17 @@ Batch::Batch(const ns_config* cfgs, int n) {
 }
 
 Batch::~Batch() {
-    dropGraphs();
-    for (void* p : allocations) (void)hipFree(p);
-    if (stream) (void)hipStreamDestroy(stream);
+    // a destructor cannot throw: queued work is waited for, failed releases are reported on stderr
+    auto report = [](hipError_t e, const char* what) {
+        if (e != hipSuccess) std::fprintf(stderr, "nsfsi: %s failed in the batch destructor: %s\n", what, hipGetErrorString(e));
+    };
+    if (stream) report(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    for (int q = 0; q < 2; ++q) {
+        if (graphExec[q]) report(hipGraphExecDestroy(graphExec[q]), "hipGraphExecDestroy");
+        graphExec[q] = nullptr;
+    }
+    for (void* p : allocations) report(hipFree(p), "hipFree");
+    if (stream) report(hipStreamDestroy(stream), "hipStreamDestroy");
 }
 
 void* Batch::alloc(size_t bytes) {

@@ -110,6 +110,10 @@ public:
     void setInflowProfile(const double* q, int64_t n);
     const double* dtSchedule = nullptr;
     int64_t dtScheduleLen = 0;
+    // Every path that advances the state calls this first: with a prescribed schedule, the step sizes of
+    // time levels level .. level + nsteps must all be in it (the finalize kernel reads entry level + 1
+    // after each step). Throws std::invalid_argument otherwise, before any step runs.
+    void checkSchedule(int64_t nsteps) const;
     double* dtScheduleBuf = nullptr;
     int64_t dtScheduleCap = 0;
     double* inflowBuf = nullptr;

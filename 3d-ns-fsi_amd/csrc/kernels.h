@@ -1,6 +1,6 @@
 /*
  * kernels.h — launch wrappers of the device kernels (declarations). All launches go to the stream
- * passed in; none synchronises.
+ * passed in; none synchronises except compactWindow (set-up only), which reads its counts back.
  */
 #pragma once
 
@@ -11,6 +11,9 @@
 #include "ns_internal.h"
 
 namespace nsfsi {
+
+// Throws a HipError (mapped to NS_ERR_HIP at the C-ABI) when e is not hipSuccess.
+void hipCheck(hipError_t e, const char* what);
 
 // Immersed body description for classification (reference Body classes, src/Boundary.h:200-346).
 struct BodyDesc {

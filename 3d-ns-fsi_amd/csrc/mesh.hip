@@ -601,17 +601,18 @@ int64_t compactWindow(hipStream_t s, const Layout& L, const uint8_t* cls, const 
     unsigned nb = blocks((int64_t)W.size(), 256);
     hipLaunchKernelGGL(k_count, dim3(nb), dim3(256), 0, s, L, cls, W, mask, value, counts);
     std::vector<int32_t> h(nb);
-    (void)hipMemcpyAsync(h.data(), counts, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
+    hipCheck(hipMemcpyAsync(h.data(), counts, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s), "copy compaction counts");
+    hipCheck(hipStreamSynchronize(s), "sync compaction counts");
     int64_t total = 0;
     for (unsigned b = 0; b < nb; ++b) {
         int32_t c = h[b];
         h[b] = (int32_t)total;
         total += c;
     }
-    (void)hipMemcpyAsync(offsets, h.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    hipCheck(hipMemcpyAsync(offsets, h.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, s), "copy compaction offsets");
     if (out && total > 0) hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, L, cls, W, mask, value, offsets, out);
-    (void)hipStreamSynchronize(s);
+    hipCheck(hipStreamSynchronize(s), "sync compaction");
+    hipCheck(hipGetLastError(), "compaction kernels");
     return total;
 }
 

@@ -156,9 +156,19 @@ Case::Case(const ns_config& c) : cfg(c) {
 }
 
 Case::~Case() {
-    dropGraphs();
-    for (void* p : allocations) (void)hipFree(p);
-    if (stream) (void)hipStreamDestroy(stream);
+    // a destructor cannot throw: work still queued on the stream is waited for, and a failed release is
+    // reported on stderr (the device memory of a failed hipFree stays with the process)
+    auto report = [](hipError_t e, const char* what) {
+        if (e != hipSuccess) std::fprintf(stderr, "nsfsi: %s failed in the case destructor: %s\n", what, hipGetErrorString(e));
+    };
+    if (stream) report(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    for (int q = 0; q < 2; ++q) {
+        if (graphExec[q]) report(hipGraphExecDestroy(graphExec[q]), "hipGraphExecDestroy");
+        graphExec[q] = nullptr;
+        graphState[q] = nullptr;
+    }
+    for (void* p : allocations) report(hipFree(p), "hipFree");
+    if (stream) report(hipStreamDestroy(stream), "hipStreamDestroy");
 }
 
 void* Case::alloc(size_t bytes) {
@@ -632,35 +642,49 @@ void Case::mark(int group, bool begin) {
 
 void Case::profile(int nsteps, double ms[4], int launches[4]) {
     if (!initialized) throw StateError("profile before init");
-    events.clear();
-    profiling = true;
-    try {
-        for (int q = 0; q < nsteps; ++q) {
-            if (fp32) stepOnce<float>();
-            else stepOnce<double>();
-        }
-    } catch (...) {
-        profiling = false;
-        throw;
-    }
-    profiling = false;
-    hipCheck(hipStreamSynchronize(stream), "sync");
+    checkSchedule(nsteps);
     double sum[4] = {0, 0, 0, 0};
     int cnt[4] = {0, 0, 0, 0};
-    for (auto& e : events) {
-        float t = 0;
-        hipCheck(hipEventElapsedTime(&t, e.a, e.b), "elapsed");
-        sum[e.group] += t;
-        cnt[e.group] += 1;
-        (void)hipEventDestroy(e.a);
-        (void)hipEventDestroy(e.b);
-    }
+    auto collect = [&](bool keep) {
+        for (auto& e : events) {
+            float t = 0;
+            if (keep && hipEventElapsedTime(&t, e.a, e.b) == hipSuccess) {
+                sum[e.group] += t;
+                cnt[e.group] += 1;
+            }
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+        events.clear();
+    };
     events.clear();
+    // batches of at most histCap steps, each followed by a drain of the norm and force rings (readScalars),
+    // as step() does, so no ring entry is overwritten before the host has it
+    for (int done = 0; done < nsteps;) {
+        const int batch = std::min(nsteps - done, histCap);
+        profiling = true;
+        try {
+            for (int q = 0; q < batch; ++q) {
+                if (fp32) stepOnce<float>();
+                else stepOnce<double>();
+            }
+        } catch (...) {
+            profiling = false;
+            (void)hipStreamSynchronize(stream);
+            collect(false);
+            throw;
+        }
+        profiling = false;
+        hipCheck(hipStreamSynchronize(stream), "sync");
+        hipCheck(hipGetLastError(), "profiled steps");
+        collect(true);
+        readScalars();
+        done += batch;
+    }
     for (int g = 0; g < 4; ++g) {
         ms[g] = cnt[g] ? sum[g] / cnt[g] : 0.0;
         launches[g] = nsteps ? cnt[g] / nsteps : 0;
     }
-    readScalars();
 }
 
 void Case::init() {
@@ -767,10 +791,14 @@ void Case::dropGraphs() {
     }
 }
 
-void Case::step(int nsteps) {
-    if (!initialized) throw StateError("step before init");
+void Case::checkSchedule(int64_t nsteps) const {
     if (dtSchedule && (int64_t)level + nsteps >= dtScheduleLen)
         throw std::invalid_argument("dt schedule too short for the requested steps");
+}
+
+void Case::step(int nsteps) {
+    if (!initialized) throw StateError("step before init");
+    checkSchedule(nsteps);
     int done = 0;
     while (done < nsteps) {
         int batch = std::min(nsteps - done, histCap);
@@ -809,6 +837,7 @@ void Case::step(int nsteps) {
 void Case::stepTaped(StageTape* tape) {
     if (!initialized) throw StateError("step before init");
     if (fp32) throw Unsupported("the adjoint runs on the FP64 form");
+    checkSchedule(1);
     tapeNow = tape;
     stepOnce<double>();  // swaps U and A itself
     tapeNow = nullptr;

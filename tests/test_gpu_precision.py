@@ -56,15 +56,22 @@ def sphere48_200():
     return out
 
 
-def test_fp32_error_vs_oracle(sphere48_200):
+# Per-field bars on the relative L2 error against the oracle after 200 steps of the 48^3 sphere: a small
+# multiple (2.5x) of the largest error measured per form (compensated combine 8.1e-7, plain 4.5e-6), so a
+# regression of either form shows, and both under the 1e-5 the FP32 form is specified to.
+FP32_BAR = {1: 2e-6, 0: 1e-5}
+
+
+@pytest.mark.parametrize("kernel", [1, 0], ids=["fused", "unfused"])
+def test_fp32_error_vs_oracle(sphere48_200, kernel):
     ref = sphere48_200["ref"]
     for kahan in (1, 0):
-        U, (t, lv, dt) = sphere48_200[(kahan, 1)]
+        U, (t, lv, dt) = sphere48_200[(kahan, kernel)]
         assert lv == 200
         assert abs(t - sphere48_200["t"]) <= 1e-5 * sphere48_200["t"]
         l2, linf = errors(U, ref)
-        print(f"kahan={kahan} rel L2 {l2} rel Linf {linf}")
-        assert np.all(l2 < 1e-4)
+        print(f"kernel={kernel} kahan={kahan} rel L2 {l2} rel Linf {linf} bar {FP32_BAR[kahan]}")
+        assert np.all(l2 < FP32_BAR[kahan]), (kahan, l2)
 
 
 @pytest.fixture(scope="module")
