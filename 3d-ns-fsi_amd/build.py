@@ -28,6 +28,7 @@ SOURCES = [
     ("batch.cpp", False),
     ("adjoint.hip", True),
     ("adjoint_host.cpp", True),
+    ("noalloc.hip", True),
 ]
 
 ARCH = "gfx950"

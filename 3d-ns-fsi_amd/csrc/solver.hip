@@ -21,6 +21,7 @@
 #include "device_common.h"
 #include "kernels.h"
 #include "stage.h"
+#include "totals.h"
 
 namespace nsfsi {
 
@@ -106,29 +107,12 @@ __global__ void k_patch_inactive(Layout L, Params P, T* prim, const int64_t* nod
 
 template <class T>
 __global__ void k_totals(Layout L, const T* S, const uint8_t* cls, double* partials) {
-    __shared__ double sh[8];
-    const size_t N = L.nodes();
-    double sm[5] = {0, 0, 0, 0, 0};
-    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(n % L.NI);
-        size_t r = n / L.NI;
-        int j = (int)(r % L.NJ), k = (int)(r / L.NJ);
-        size_t o = L.off(i, j, k);
-        if ((cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) continue;
-        for (int q = 0; q < 5; ++q) sm[q] += (double)S[o + q * L.fstride];
-    }
-    for (int q = 0; q < 5; ++q) {
-        double v = wave_sum(sm[q]);
-        int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        __syncthreads();
-        if (lane == 0) sh[wv] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = 0;
-            for (int b = 0; b < (int)(blockDim.x >> 6); ++b) r += sh[b];
-            partials[(size_t)blockIdx.x * 5 + q] = r;
-        }
-    }
+    auto offOf = [&](size_t n) {
+        const int i = (int)(n % L.NI);
+        const size_t r = n / L.NI;
+        return L.off(i, (int)(r % L.NJ), (int)(r / L.NJ));
+    };
+    totalsBlock<T>(L.nodes(), offOf, L.fstride, S, cls, partials);
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -966,11 +950,13 @@ int64_t Case::interiorCount() {
 }
 
 void Case::totals(double out[5]) {
-    int nb = 256;
+    const int nb = kTotalsBlocks;
     double* part = partials;  // reuse (1024 x 10 doubles >= 256 x 5)
     hipCheck(hipStreamSynchronize(stream), "sync");
-    if (fp32) hipLaunchKernelGGL(k_totals<float>, dim3(nb), dim3(256), 0, stream, L, (const float*)U, cls, part);
-    else hipLaunchKernelGGL(k_totals<double>, dim3(nb), dim3(256), 0, stream, L, (const double*)U, cls, part);
+    if (fp32)
+        hipLaunchKernelGGL(k_totals<float>, dim3(nb), dim3(kTotalsThreads), 0, stream, L, (const float*)U, cls, part);
+    else
+        hipLaunchKernelGGL(k_totals<double>, dim3(nb), dim3(kTotalsThreads), 0, stream, L, (const double*)U, cls, part);
     std::vector<double> h((size_t)nb * 5);
     hipCheck(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream), "copy totals");
     hipCheck(hipStreamSynchronize(stream), "sync");

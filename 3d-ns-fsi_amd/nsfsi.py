@@ -119,6 +119,11 @@ def lib():
     L.ns_destroy.argtypes = [H]
     L.ns_destroy.restype = None
     L.ns_abi_version.restype = C.c_int32
+    L.ns_totals_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                    C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+    L.ns_totals_noalloc.restype = C.c_int
+    L.ns_noalloc_last_error.argtypes = []
+    L.ns_noalloc_last_error.restype = C.c_char_p
     B = C.c_void_p
     L.ns_batch_create.argtypes = [C.POINTER(NSConfig), C.c_int32, C.POINTER(B)]
     L.ns_batch_init.argtypes = [B]

@@ -273,6 +273,21 @@ ns_status ns_batch_get_tile(ns_batch* b, int32_t* ty, int32_t* kc);
 const char* ns_batch_last_error(ns_batch* b);
 void ns_batch_destroy(ns_batch* b);
 
+/* ---- Allocation-free entry point.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same result as ns_totals, bit for bit, on
+ * fields the caller holds in device memory: fields = 5 consecutive dense x-fastest NI*NJ*NK arrays (rho',
+ * rho*u, rho*v, rho*w, rho*E') of double (precision NS_FP64) or float (NS_FP32); node_types = NI*NJ*NK bytes
+ * holding ns_node_type values, the layout ns_get_node_types writes, in device memory; spacing = dx, dy, dz;
+ * out = 5 doubles of device memory receiving the sums over NS_FLUID_INTERIOR nodes times dx*dy*dz.
+ * stream: a hipStream_t (null: the default stream); the call only enqueues work there and returns.
+ * It allocates nothing: its scratch is a fixed 256 x 5 doubles held by the library on each GPU for the life of
+ * the process, shared by every call on that GPU, so calls that overlap on one GPU race. Grids beyond 2^36
+ * nodes are refused with NS_ERR_ARGUMENT; the text of the last error is ns_noalloc_last_error(). */
+ns_status ns_totals_noalloc(const void* fields, int32_t precision, const uint8_t* node_types, int32_t NI, int32_t NJ,
+                            int32_t NK, const double spacing[3], double* out, void* stream);
+const char* ns_noalloc_last_error(void);
+
 /* Library ABI version, NS_FSI_ABI_VERSION. */
 int32_t ns_abi_version(void);
 
