@@ -90,7 +90,7 @@ public:
     // fields (element type float or double)
     void *U = nullptr, *A = nullptr, *S1 = nullptr, *S2 = nullptr, *prim = nullptr;
     void *rhoPre = nullptr, *rhoEPre = nullptr, *ftmp = nullptr;
-    uint16_t* comp = nullptr;  // FP32 compensated combine: carried rounding error, 5 bfloat16 fields
+    int8_t* comp = nullptr;  // FP32 compensated combine: carried rounding error, 5 fields of 8 bits
 
     // step scalars and reductions
     StepScalars* scal = nullptr;

@@ -79,15 +79,25 @@ __device__ __forceinline__ void outletValues(T gm1, const T a[4], const T b[4], 
 
 // Compensated FP32 combine. The increment form keeps Uⁿ out of the stage accumulator (it holds
 // D = dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4 only), so the one rounding that is relative to |U| happens
-// in U + D at the end of the step. TwoSum (Knuth) gives that rounding error exactly; it is carried to
-// the next step's increment (Kahan's compensated summation over the time steps) in a bfloat16 buffer:
-// the error is at most half an ulp of U, and 8 significant bits of it leave a residual of 2^-9 of that.
-__device__ __forceinline__ float bf16Decode(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ __forceinline__ uint16_t bf16Encode(float x) {
-    // round to nearest, ties to even (the compensation is finite wherever the state is)
-    uint32_t b = __float_as_uint(x);
-    b += 0x7fffu + ((b >> 16) & 1u);
-    return (uint16_t)(b >> 16);
+// in U + D at the end of the step. TwoSum (Knuth) gives that rounding error e exactly; it is carried to
+// the next step's increment (Kahan's compensated summation over the time steps) as a signed 8-bit
+// multiple of ulp(s)/254, s the rounded sum it belongs to (|e| <= ulp(s)/2, so the multiple is at most
+// 127): the residual is at most ulp(s)/508, about 2^-9 of the FP32 rounding. Sums below 2^-103 in
+// magnitude take the scale of 2^-103 (their error then encodes as 0). The next step decodes it with its
+// own copy of s, the state it reads.
+__device__ __forceinline__ int carryExp(float s) {
+    const int E = (int)((__float_as_uint(s) >> 23) & 0xffu);
+    return E < 24 ? 24 : E;
+}
+__device__ __forceinline__ int8_t carryEncode(float e, float s) {
+    const float inv = __uint_as_float((uint32_t)(277 - carryExp(s)) << 23);  // 1 / ulp(s)
+    float q = rintf(e * inv * 254.0f);
+    q = fminf(fmaxf(q, -127.0f), 127.0f);
+    return (int8_t)(int)q;
+}
+__device__ __forceinline__ float carryDecode(int8_t q, float s) {
+    const float ulp = __uint_as_float((uint32_t)(carryExp(s) - 23) << 23);
+    return (float)q * (ulp * (1.0f / 254.0f));
 }
 // s + e == u + d exactly (s = fl(u + d)); no operand order assumption.
 __device__ __forceinline__ float twoSum(float u, float d, float& e) {

@@ -7,7 +7,7 @@
  * segment is then 256 B (FP32) or 512 B (FP64) and starts on such a boundary: the fused stage kernel,
  * whose x-tiles are the segments, writes whole aligned cache lines (a line written in part costs a
  * read-modify-write in the memory system). Every x-row is padded (pitch) to whole segments in FP32 (so the
- * bfloat16 compensation carry, at half the byte offsets, also starts each row on a 128-byte line) and to a
+ * 8-bit compensation carry, at a quarter of the byte offsets, takes 64 aligned bytes per segment) and to a
  * multiple of 128 bytes in FP64; every field starts on a 256-byte boundary;
  * offset(i,j,k) = (k*NJ + j)*pitch + col(i). The five
  * conserved fields of one state (rho', rho*u, rho*v, rho*w, rho*E') are consecutive fields of one

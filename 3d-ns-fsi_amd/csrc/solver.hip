@@ -36,8 +36,8 @@ Layout makeLayout(int NI, int NJ, int NK, size_t eb) {
     L.NI = NI;
     L.NJ = NJ;
     L.NK = NK;
-    // FP32 rows hold whole segments (256 B), so the bfloat16 compensation carry, stored at half the byte
-    // offsets, also starts every row on a 128-byte line; FP64 rows are padded to 128 B
+    // FP32 rows hold whole segments (256 B), so the 8-bit compensation carry, stored at a quarter of the byte
+    // offsets, covers 64 aligned bytes per segment; FP64 rows are padded to 128 B
     const size_t rowElems = eb == 4 ? (size_t)Layout::kSegSlots : 128 / eb;
     const size_t slots = (size_t)Layout::col(NI - 1) + 1;  // through the last node's slot
     L.pitch = (int)((slots + rowElems - 1) / rowElems * rowElems);
@@ -317,7 +317,7 @@ void Case::allocFields() {
     S1 = alloc(fb);
     S2 = alloc(fb);
     prim = alloc(7 * L.fstride * sizeof(T));
-    if (fp32 && cfg.kahan) comp = (uint16_t*)alloc(5 * L.fstride * sizeof(uint16_t));
+    if (fp32 && cfg.kahan) comp = (int8_t*)alloc(5 * L.fstride * sizeof(int8_t));
     hipCheck(hipMemsetAsync(U, 0, fb, stream), "memset");
     hipCheck(hipMemsetAsync(A, 0, fb, stream), "memset");  // the reference's zero-initialised "Old" arrays
     hipCheck(hipMemsetAsync(S1, 0, fb, stream), "memset");
@@ -398,7 +398,7 @@ void Case::hostInitialField(std::vector<T>& h) const {
 template <class T>
 void Case::uploadState(const std::vector<T>& h) {
     hipCheck(hipMemcpyAsync(U, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream), "upload IC");
-    if (comp) hipCheck(hipMemsetAsync(comp, 0, 5 * L.fstride * sizeof(uint16_t), stream), "memset");
+    if (comp) hipCheck(hipMemsetAsync(comp, 0, 5 * L.fstride * sizeof(int8_t), stream), "memset");
     // pre-filter side tables start as the current values (the reference's primitives match the IC)
     launchFilterCompute<T>(stream, L, W, filterList, nFilter, (const T*)U, (T*)ftmp, (T*)rhoPre, (T*)rhoEPre);
     if (nInactive > 0)

@@ -37,7 +37,7 @@ void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T
 template <class T>
 void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                     T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* filteredRho,
-                    const int32_t* filterFlag, uint16_t* comp);
+                    const int32_t* filterFlag, int8_t* comp);
 
 // Periodic density filter (Mesh::applyFilter_ifAppropriate, src/Mesh.cpp:108-140), in three parts:
 // the flag for this step from (t, timeLevel); the filtered interior density of U into `out`; after the
@@ -67,14 +67,15 @@ constexpr int kMinTileKc = 8;  // smallest z-chunk accepted (sizes the scan part
 // the stage input's density, and non-interior nodes copy their density from the stage input.
 // comp != null (FP32 only; both stage kernels): compensated combine. The accumulator A holds the step's
 // increment at interior nodes (stages 1-3) and stage 4 forms U + increment with the carried rounding
-// error; comp is 5 bfloat16 fields with the layout's element offsets, zero before the first step.
+// error; comp is 5 fields of 8-bit carries (carryEncode) with the layout's element offsets, zero before
+// the first step.
 // outlet != null: that outlet face (outletFusable) is applied to the stage output (Sout, A in stage 4) by
 // the kernel itself, with the values the edge-face pass would give it when run first after the kernel.
 template <class T>
 int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
                      const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
-                     const int32_t* filterFlag, uint16_t* comp, const EdgeFace* outlet = nullptr);
+                     const int32_t* filterFlag, int8_t* comp, const EdgeFace* outlet = nullptr);
 // Whether the fused stage kernel can apply edge face f itself: an outlet at x = NI-1 whose row's last
 // x-segment holds at least three nodes (the node and the two it extrapolates from share a wavefront).
 // The caller also keeps f out of the edge-face pass and checks that f is independent of every face
@@ -91,7 +92,7 @@ const int* fusedChunkPlanes(int* n);  // z-chunk lengths the autotuners try
 size_t fusedArgsBytes();
 int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const Window& W, const float* Sin,
                    const float* U, float* Sout, float* A, const uint8_t* cls, const float* rhoPre, const float* rhoEPre,
-                   const double* dtdev, FusedTile tile, double* partials, int norms, uint16_t* comp,
+                   const double* dtdev, FusedTile tile, double* partials, int norms, int8_t* comp,
                    const EdgeFace* outlet = nullptr);
 void launchStageBatched(hipStream_t s, int stage, int ty, bool scan, bool kahan, const void* tab, const int* start,
                         int ncases, int total);

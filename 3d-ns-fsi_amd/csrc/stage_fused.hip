@@ -102,9 +102,9 @@ struct FusedArgs {
     // stage 1 of a step with the periodic density filter: combine base density and flag
     const T* fRho;
     const int32_t* fFlag;
-    // compensated FP32 combine: per-field bfloat16 rounding error of U carried between steps (the
-    // stage accumulator then holds the increment only)
-    uint16_t* comp[5];
+    // compensated FP32 combine: per-field 8-bit rounding error of U carried between steps (the stage
+    // accumulator then holds the increment only)
+    int8_t* comp[5];
 };
 
 // Whole-wavefront shifts by one lane (DPP wave_shl:1 / wave_shr:1 of GFX9): shl1 gives each lane the
@@ -167,7 +167,7 @@ struct Mem<float, true> {
         u = rsrc(x.U[0], nb);
         a = rsrc(x.A[0], nb);
         sout = rsrc(x.Sout[0], nb);
-        comp = rsrc(x.comp[0], nb / 2);
+        comp = rsrc(x.comp[0], nb / 4);
     }
     __device__ float ld(__amdgpu_buffer_rsrc_t r, int q, uint32_t bo) const {
         return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)bo, (int)(q * fsb), 0));
@@ -182,12 +182,12 @@ struct Mem<float, true> {
     __device__ float ldA(int q, uint32_t bo) const { return ld(a, q, bo); }
     __device__ void stA(int q, uint32_t bo, float v) const { st(a, q, bo, v); }
     __device__ void stSout(int q, uint32_t bo, float v) const { st(sout, q, bo, v); }
-    // compensation: bfloat16 fields at half the byte offsets
-    __device__ uint16_t ldComp(int q, uint32_t bo) const {
-        return __builtin_amdgcn_raw_buffer_load_b16(comp, (int)(bo >> 1), (int)(q * (fsb >> 1)), 0);
+    // compensation: 8-bit fields at a quarter of the byte offsets
+    __device__ int8_t ldComp(int q, uint32_t bo) const {
+        return (int8_t)__builtin_amdgcn_raw_buffer_load_b8(comp, (int)(bo >> 2), (int)(q * (fsb >> 2)), 0);
     }
-    __device__ void stComp(int q, uint32_t bo, uint16_t v) const {
-        __builtin_amdgcn_raw_buffer_store_b16(v, comp, (int)(bo >> 1), (int)(q * (fsb >> 1)), kNT);
+    __device__ void stComp(int q, uint32_t bo, int8_t v) const {
+        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, comp, (int)(bo >> 2), (int)(q * (fsb >> 2)), kNT);
     }
 };
 
@@ -200,12 +200,12 @@ struct Mem<T, false> {
     __device__ T ldA(int q, uint32_t bo) const { return ldAt((const T*)x.A[q], bo); }
     __device__ void stA(int q, uint32_t bo, T v) const { stNT(x.A[q], bo, v); }
     __device__ void stSout(int q, uint32_t bo, T v) const { stNT(x.Sout[q], bo, v); }
-    __device__ uint16_t ldComp(int q, uint32_t bo) const {
-        if constexpr (sizeof(T) == 4) return ldAt((const uint16_t*)x.comp[q], bo >> 1);
+    __device__ int8_t ldComp(int q, uint32_t bo) const {
+        if constexpr (sizeof(T) == 4) return ldAt((const int8_t*)x.comp[q], bo >> 2);
         else return 0;
     }
-    __device__ void stComp(int q, uint32_t bo, uint16_t v) const {
-        if constexpr (sizeof(T) == 4) stNT(x.comp[q], bo >> 1, v);
+    __device__ void stComp(int q, uint32_t bo, int8_t v) const {
+        if constexpr (sizeof(T) == 4) stNT(x.comp[q], bo >> 2, v);
     }
 };
 
@@ -314,12 +314,20 @@ __device__ __forceinline__ uint32_t slotBytes(const Layout& L, int s, int j) {
     return (uint32_t)(j * L.pitch + s) * (uint32_t)sizeof(T);
 }
 
+// Planes past `last` (the highest plane the tile's stencil reads: one past its last output plane) are not
+// loaded: the software pipeline runs ahead of the loop's end and those values are never used.
 template <class T, class M>
-__device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, const M& m, int ic, int jc, uint32_t col, int k) {
+__device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, const M& m, int ic, int jc, uint32_t col, int k,
+                                        int last) {
     const Layout& L = a.L;
+    Raw<T> r;
+    if (k > last) {  // uniform over the block
+        r.rho = r.ru = r.rv = r.rw = r.E = r.rp = r.Ep = 0;
+        r.filt = false;
+        return r;
+    }
     const int kc = min(max(k, 0), L.NK - 1);
     const uint32_t bo = col + (uint32_t)kc * (uint32_t)(L.plane * sizeof(T));
-    Raw<T> r;
     r.rho = m.ldSin(0, bo);
     r.ru = m.ldSin(1, bo);
     r.rv = m.ldSin(2, bo);
@@ -461,9 +469,9 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     const uint32_t planeB = (uint32_t)(L.plane * sizeof(T));
     // Slots without a node that this wave writes (values never read): the segment's padding slots and, in
     // FP32, in the row's last segment, the slots past the last node up to the end of its 128-byte line
-    // (for the bfloat16 carry, whose line spans a whole segment, up to the segment's end; a half-width
-    // tile writes the carry's upper half with a second store), so that no line of the row is written in
-    // part. (FP64 ran 0.5 % slower with its last line completed on 512-node rows, whose last line holds
+    // (for the 8-bit carry, whose 64 aligned bytes span a whole segment, up to the segment's end; a
+    // half-width tile writes the carry's upper half with a second store), so that no line (carry: no
+    // aligned 64-byte half) of the row is written in part. (FP64 ran 0.5 % slower with its last line completed on 512-node rows, whose last line holds
     // one node and fifteen slots past it.)
     const int slot = Layout::kSegSlots * bx + sl, endSlot = Layout::col(L.NI - 1) + 1;
     const bool padLane = sl == 0 || sl == 63 || i >= L.NI;
@@ -474,18 +482,19 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
 
     // prologue: planes kb-1 and kb, and the fetch of kb+1 in flight
     Pt<T> qm, q0, q1;
-    Raw<T> rnext = fetch(a, m, ic, jc, colc, kb - 1);
+    const int last = ke;  // highest plane the stencil of output plane ke-1 reads
+    Raw<T> rnext = fetch(a, m, ic, jc, colc, kb - 1, last);
     const bool inner = row >= 1 && row <= TY;  // output rows (halo rows are 0 and TY+1)
     finishPlane<T, TY>(a, smb[0], row, lane, rnext, qm, inner);
-    rnext = fetch(a, m, ic, jc, colc, kb);
+    rnext = fetch(a, m, ic, jc, colc, kb, last);
     if (NB == 1) __syncthreads();  // the readers of plane kb-1 are done before its buffer is rewritten
     finishPlane<T, TY>(a, smb[NB - 1], row, lane, rnext, q0, inner);
-    rnext = fetch(a, m, ic, jc, colc, kb + 1);
+    rnext = fetch(a, m, ic, jc, colc, kb + 1, last);
     // The stencil input is fetched two planes ahead of its use, except in the FP32 stage 4, whose
     // registers go to the scans and the compensation (two planes ahead spill there).
     constexpr bool kDeep = STAGE != 4 || sizeof(T) == 8;
     Raw<T> rnext2{};
-    if constexpr (kDeep) rnext2 = fetch(a, m, ic, jc, colc, kb + 2);
+    if constexpr (kDeep) rnext2 = fetch(a, m, ic, jc, colc, kb + 2, last);
     T gz0[5], phizm[5];
     if (inner) {
         T gzm[5], cz[5];
@@ -513,9 +522,9 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
         const Raw<T> rcur = rnext;
         if constexpr (kDeep) {
             rnext = rnext2;
-            rnext2 = fetch(a, m, ic, jc, colc, k + 3);
+            rnext2 = fetch(a, m, ic, jc, colc, k + 3, last);
         } else {
-            rnext = fetch(a, m, ic, jc, colc, k + 2);
+            rnext = fetch(a, m, ic, jc, colc, k + 2, last);
         }
         // combine inputs of the output node, fetched before the stencil work
         const bool doOut = outRow && outLane;
@@ -600,7 +609,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
             // so a wave whose nodes all write covers its segment's lines in full.
             T cur[5] = {q0.rho, q0.ru, q0.rv, q0.rw, q0.E};
             T vS[5], vA[5];
-            uint16_t vC[5];
+            int8_t vC[5];
             bool wS = padLine, wA = padLine, wC = padSeg;
             for (int q = 0; q < 5; ++q) {
                 vS[q] = vA[q] = cur[q];
@@ -633,8 +642,8 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                             if constexpr (KAHAN) {
                                 // U^{n+1} = U^n + (D + e^{n-1}), its rounding error kept for the next step
                                 T e;
-                                an[q] = twoSum(base[q], an[q] + bf16Decode(m.ldComp(q, bo)), e);
-                                vC[q] = bf16Encode(e);
+                                an[q] = twoSum(base[q], an[q] + carryDecode(m.ldComp(q, bo), base[q]), e);
+                                vC[q] = carryEncode(e, an[q]);
                             }
                             vA[q] = an[q];
                         }
@@ -954,7 +963,7 @@ int normTY(int ty) { return ty == 4 || ty == 6 || ty == 14 ? ty : 8; }
 template <class T>
 void fillArgs(FusedArgs<T>& a, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin, const T* U,
               T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre, const double* dtdev, FusedTile tile,
-              double* partials, int norms, const T* filteredRho, const int32_t* filterFlag, uint16_t* comp,
+              double* partials, int norms, const T* filteredRho, const int32_t* filterFlag, int8_t* comp,
               const EdgeFace* outlet) {
     a = FusedArgs<T>{};
     if (comp && sizeof(T) != 4) throw std::invalid_argument("compensated combine needs FP32 fields");
@@ -1001,7 +1010,7 @@ template <class T>
 int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin,
                      const T* U, T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre,
                      const double* dtdev, FusedTile tile, double* partials, int norms, const T* filteredRho,
-                     const int32_t* filterFlag, uint16_t* comp, const EdgeFace* outlet) {
+                     const int32_t* filterFlag, int8_t* comp, const EdgeFace* outlet) {
     FusedArgs<T> a;
     fillArgs(a, stage, L, P, W, Sin, U, Sout, A, cls, rhoPre, rhoEPre, dtdev, tile, partials, norms, filteredRho,
              filterFlag, comp, outlet);
@@ -1019,7 +1028,7 @@ size_t fusedArgsBytes() { return sizeof(FusedArgs<float>); }
 
 int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const Window& W, const float* Sin,
                    const float* U, float* Sout, float* A, const uint8_t* cls, const float* rhoPre, const float* rhoEPre,
-                   const double* dtdev, FusedTile tile, double* partials, int norms, uint16_t* comp,
+                   const double* dtdev, FusedTile tile, double* partials, int norms, int8_t* comp,
                    const EdgeFace* outlet) {
     if (!stateFitsBuffer(L)) throw std::length_error("batched case: state buffer beyond 4 GiB");
     FusedArgs<float> a;
@@ -1089,10 +1098,10 @@ const int* fusedChunkPlanes(int* n) {
 template int launchStageFused<double>(hipStream_t, int, const Layout&, const Params&, const Window&, const double*,
                                       const double*, double*, double*, const uint8_t*, const double*, const double*,
                                       const double*, FusedTile, double*, int, const double*, const int32_t*,
-                                      uint16_t*, const EdgeFace*);
+                                      int8_t*, const EdgeFace*);
 template int launchStageFused<float>(hipStream_t, int, const Layout&, const Params&, const Window&, const float*,
                                      const float*, float*, float*, const uint8_t*, const float*, const float*,
                                      const double*, FusedTile, double*, int, const float*, const int32_t*,
-                                     uint16_t*, const EdgeFace*);
+                                     int8_t*, const EdgeFace*);
 
 }  // namespace nsfsi

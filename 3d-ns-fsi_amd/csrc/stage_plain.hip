@@ -78,7 +78,7 @@ __global__ void k_derive(Layout L, Params P, const T* S, T* prim, const uint8_t*
 
 template <class T, int STAGE, bool KAHAN>
 __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, const T* U, T* Sout, T* A, const uint8_t* cls,
-                            const double* dtdev, const T* fRho, const int32_t* fFlag, uint16_t* comp) {
+                            const double* dtdev, const T* fRho, const int32_t* fFlag, int8_t* comp) {
     const bool rhoFiltered = STAGE == 1 && fFlag && *fFlag;
     const int j = blockIdx.y, k = blockIdx.z;
     const SegSlot sl = segSlot(L, j, k);
@@ -224,8 +224,8 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
             if constexpr (KAHAN) {
                 // U^{n+1} = U^n + (D + e^{n-1}); the new rounding error is carried (device_common.h)
                 T e;
-                an = twoSum(U[o + q * F], an + bf16Decode(comp[o + q * F]), e);
-                comp[o + q * F] = bf16Encode(e);
+                an = twoSum(U[o + q * F], an + carryDecode(comp[o + q * F], U[o + q * F]), e);
+                comp[o + q * F] = carryEncode(e, an);
             }
             A[o + q * F] = an;
         }
@@ -244,7 +244,7 @@ void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T
 template <class T, bool K>
 void launchRhsPlainK(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                      T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag,
-                     uint16_t* comp) {
+                     int8_t* comp) {
     dim3 blk(Layout::kSegSlots), grd((L.NI + Layout::kSegNodes - 1) / Layout::kSegNodes, L.NJ, L.NK);
     switch (stage) {
         case 1:
@@ -269,7 +269,7 @@ void launchRhsPlainK(hipStream_t s, int stage, const Layout& L, const Params& P,
 template <class T>
 void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                     T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag,
-                    uint16_t* comp) {
+                    int8_t* comp) {
     if constexpr (sizeof(T) == 4) {
         if (comp) {
             launchRhsPlainK<T, true>(s, stage, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag, comp);
@@ -351,9 +351,9 @@ template void launchDerive<float>(hipStream_t, const Layout&, const Params&, con
                                   const Window&, const float*, const float*);
 template void launchRhsPlain<double>(hipStream_t, int, const Layout&, const Params&, const double*, const double*,
                                      const double*, double*, double*, const uint8_t*, const double*, const double*,
-                                     const int32_t*, uint16_t*);
+                                     const int32_t*, int8_t*);
 template void launchRhsPlain<float>(hipStream_t, int, const Layout&, const Params&, const float*, const float*,
                                     const float*, float*, float*, const uint8_t*, const double*, const float*,
-                                    const int32_t*, uint16_t*);
+                                    const int32_t*, int8_t*);
 
 }  // namespace nsfsi
