@@ -22,7 +22,7 @@ SOURCES = [
     ("capi.cpp", True),
     ("mesh.hip", True),
     ("solver.hip", True),
-    ("stage_plain.hip", False),
+    ("stage_plain.hip", True),
     ("stage_fused.hip", False),
     ("reduce.hip", False),
     ("batch.cpp", False),
@@ -74,7 +74,30 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_host_programs()
     return LIB
+
+
+# Host programs over the C-ABI, compiled with g++ against libnsfsi.so (found next to them, or through the
+# rpath): the command-line program of the reference's main, and the drop-in test program of the test suite.
+ROOT = os.path.dirname(HERE)
+HOST_PROGRAMS = [
+    (os.path.join(HERE, "host", "main.cpp"), os.path.join(HERE, "nsfsi_run")),
+    (os.path.join(ROOT, "tests", "cpp", "solver_dropin.cpp"), os.path.join(ROOT, "tests", "cpp", "solver_dropin")),
+]
+
+
+def build_host_programs():
+    for src, out in HOST_PROGRAMS:
+        if not os.path.exists(src):
+            continue
+        rpath = os.path.relpath(HERE, os.path.dirname(out))
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-o", out + ".tmp",
+               "-L", HERE, "-lnsfsi", f"-Wl,-rpath,$ORIGIN/{rpath}", "-Wl,-rpath-link,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host program build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

@@ -203,6 +203,49 @@ def make_config(**kw) -> NSConfig:
     return c
 
 
+def config_text(c: NSConfig) -> str:
+    """A ConfigFile (libconfig syntax, the reference's keys plus the extension keys) that ns_config_parse reads
+    back into the same configuration: integers without a decimal point, floats with one (libconfig's strict
+    typing), floats written with 17 significant digits."""
+    def f(x):
+        r = repr(float(x))
+        return r if ("." in r or "e" in r or "n" in r) else r + "."
+
+    inv_bc = {v: k for k, v in BC.items()}
+    inv_body = {v: k for k, v in BODY.items()}
+    inv_ic = {v: k for k, v in IC.items()}
+    b = lambda x: "true" if x else "false"  # noqa: E731
+    lines = [
+        f"NI = {c.NI}", f"NJ = {c.NJ}", f"NK = {c.NK}",
+        f'stopCriterion = "{"timesteps" if c.stop_criterion == 0 else "end_time"}"',
+        f"stopTimeLevel = {c.stop_time_level}", f"t_end = {f(c.t_end)}",
+        f"convStabilityLimit = {f(c.conv_stability_limit)}", f"viscStabilityLimit = {f(c.visc_stability_limit)}",
+        "filterIntervals = [" + ", ".join(str(c.filter_intervals[q]) for q in range(c.n_filter_intervals)) + "]",
+        "filterIntervalChangeTimes = [" + ", ".join(f(c.filter_change_times[q]) for q in range(c.n_filter_change_times))
+        + "]",
+        f"statusReportInterval = {f(c.status_report_interval)}",
+        f"saveDensity = {b(c.save_density)}", f"saveMomentum = {b(c.save_momentum)}",
+        f"saveEnergy = {b(c.save_energy)}", f"saveVelocity = {b(c.save_velocity)}",
+        f"savePressure = {b(c.save_pressure)}", f"saveTemperature = {b(c.save_temperature)}",
+        f"saveViscosity = {b(c.save_viscosity)}", f"saveThermalCond = {b(c.save_thermal_cond)}",
+        f"saveIC = {b(c.save_ic)}", f"saveFinal = {b(c.save_final)}", f"saveIntervals = {b(c.save_intervals)}",
+        f"savePeriod = {f(c.save_period)}", f"saveIntervalsStartTime = {f(c.save_intervals_start_time)}",
+        f"saveIntervalsEndTime = {f(c.save_intervals_end_time)}",
+        f'saveConvergenceHistory = "{["none", "density", "all"][c.conv_history]}"',
+        f"Gamma = {f(c.gamma)}", f"Pr = {f(c.Pr)}", f"R = {f(c.R)}", f"Re = {f(c.Re)}",
+        f"sutherlands_C2 = {f(c.sutherlands_C2)}", f"M_0 = {f(c.M_0)}", f"T_0 = {f(c.T_0)}",
+        f"L_x = {f(c.L_x)}", f"L_y = {f(c.L_y)}", f"L_z = {f(c.L_z)}",
+        "boundaryConditions = [" + ", ".join(f'"{inv_bc[c.bc[q]]}"' for q in range(6)) + "]",
+        f'bodyType = "{inv_body[c.body_type]}"', f'bodyAxis = "{"xyz"[c.body_axis]}"',
+        "bodyCentre = [" + ", ".join(f(c.body_centre[q]) for q in range(3)) + "]",
+        f"bodyRadius = {f(c.body_radius)}", f'initialCondition = "{inv_ic[c.ic_type]}"',
+        f"perturbationSeed = {c.ic_seed}", f"perturbationAmplitude = {f(c.ic_amplitude)}",
+        f"inletRampTime = {f(c.inlet_ramp_time)}", f'precision = "{"fp64" if c.precision == 0 else "fp32"}"',
+        f"kahan = {b(c.kahan)}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
 def parse_config(text: str) -> NSConfig:
     """ConfigSettings::loadSettings on a ConfigFile text (src/ConfigSettings.cpp:18-28)."""
     c = NSConfig()
