@@ -24,7 +24,7 @@ SOURCES = [
     ("solver.hip", True),
     ("stage_plain.hip", True),
     ("stage_fused.hip", False),
-    ("reduce.hip", False),
+    ("reduce.hip", True),
     ("batch.cpp", False),
     ("adjoint.hip", True),
     ("adjoint_host.cpp", True),

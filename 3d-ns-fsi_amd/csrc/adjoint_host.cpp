@@ -81,6 +81,7 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
             hipCheck(hipMemcpyAsync(scal, &c.s, sizeof(StepScalars), hipMemcpyHostToDevice, stream), "copy");
             hipCheck(hipStreamSynchronize(stream), "sync");
             readScalars();
+            primsStored = false;  // stored primitive fields (unfused FP64 form) are derived again from the state
         };
         bodyForce = true;
         forceHost.clear();

@@ -49,6 +49,16 @@ public:
     // Run-time options: "stage_kernel" 0 = unfused (primitive pass + per-node RHS), 1 = fused.
     void setOption(const std::string& name, int64_t value);
     int stageKernel = 1;
+    // The FP64 unfused stage keeps the reference's stored primitive and transport fields (`prim`): interior
+    // nodes updated after each stage combine, the other nodes written by the boundary conditions, untouched
+    // solid nodes at their initial values, and the reference's own final combine. Bit for bit the
+    // reference's arithmetic, given a correctly rounded x^1.5.
+    bool literal() const { return stageKernel == 0 && !fp32; }
+    // `prim` holds the stored values (set by the initial state, kept by stored-primitive steps; other steps
+    // and primitive read-outs in the other forms leave it stale, and the next stored-primitive step first
+    // derives it from the current state)
+    bool primsStored = false;
+    void syncStoredPrims();
     bool fusedScans = true;  // end-of-step scans inside the stage-4 kernel (else a separate full pass)
     FusedTile tile{};
     // Times every fused-stage tile shape on this case (stage-2 launches into scratch outputs, state
@@ -143,7 +153,7 @@ private:
     template <class T> void hostInitialField(std::vector<T>& h) const;
     template <class T> void uploadState(const std::vector<T>& h);
     template <class T> void computeDt(int mode);
-    template <class T> void applyBCs(T* S, double frac, bool outletDone);
+    template <class T> void applyBCs(T* S, double frac, bool outletDone, T* storedPrims = nullptr);
     template <class T> void stepOnce();
     std::vector<void*> allocations;
     StageTape* tapeNow = nullptr;

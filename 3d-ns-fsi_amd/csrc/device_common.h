@@ -17,6 +17,7 @@ template <class T>
 struct Eos {
     T gamma, gm1, ScPlusOne, Re, pf;  // pf = 1 / ((gamma-1) Pr)
     T invGm1;                         // 1 / (gamma-1)
+    T gm1Pr;                          // (gamma-1) Pr
     __host__ __device__ static Eos make(const Params& P) {
         Eos e;
         e.gamma = (T)P.gamma;
@@ -25,12 +26,24 @@ struct Eos {
         e.Re = (T)P.Re;
         e.pf = (T)(1 / ((P.gamma - 1) * P.Pr));
         e.invGm1 = (T)(1 / (P.gamma - 1));
+        e.gm1Pr = (T)((P.gamma - 1) * P.Pr);
         return e;
     }
 };
 
-// (1+T')^1.5 as x*sqrt(x); pow(x, 1.5) of the reference (src/Solver.cpp:476) differs by <= 1 ulp.
-__device__ __forceinline__ double pow15(double x) { return x * sqrt(x); }
+// (1+T')^1.5 = x^1.5 (src/Solver.cpp:476, src/FlowVariableGroupStructs.h:183), correctly rounded in FP64:
+// with s = sqrt(x) (correctly rounded), the residual x - s^2 is exact in one FMA, so sqrt(x) = s + e with
+// e = (x - s^2)/(2s) to within 2^-53 |e|; x*s = p + pe exactly (FMA); x^1.5 = p + (pe + x*e) then carries
+// one final rounding of an error of about 2^-104 |p| (Dekker's double-length products). FP32: x*sqrt(x).
+__device__ __forceinline__ double pow15(double x) {
+#pragma clang fp contract(off)
+    const double s = sqrt(x);
+    const double r = __builtin_fma(-s, s, x);
+    const double e = r / (2.0 * s);
+    const double p = x * s;
+    const double pe = __builtin_fma(x, s, -p);
+    return p + (pe + x * e);
+}
 __device__ __forceinline__ float pow15(float x) { return x * sqrtf(x); }
 
 // Primitive variables and viscosity from conserved (updatePrimitiveVariables, src/Solver.cpp:446-479).
@@ -44,6 +57,15 @@ __device__ __forceinline__ void derivePrims(const Eos<T>& e, T rho, T ru, T rv, 
     p = e.gm1 * (rE - m / (T)2 * (u * u + v * v + w * w));
     Tt = (e.gamma * p - rho) / ((T)1 + rho);
     mu = pow15((T)1 + Tt) * e.ScPlusOne / (e.Re * (Tt + e.ScPlusOne));
+}
+
+// Transport properties from the temperature as the boundary conditions and initial condition set them
+// (deriveTransportProperties, src/FlowVariableGroupStructs.h:176-185: kappa = mu / ((gamma-1) Pr), a division,
+// where the stage update of interior nodes multiplies by 1 / ((gamma-1) Pr), src/Solver.cpp:451, 478).
+template <class T>
+__device__ __forceinline__ void deriveTrans(const Eos<T>& e, T Tt, T& mu, T& kappa) {
+    mu = pow15((T)1 + Tt) * e.ScPlusOne / (e.Re * (Tt + e.ScPlusOne));
+    kappa = mu / e.gm1Pr;
 }
 
 // Conserved from primitive (deriveConservedVariables, src/FlowVariableGroupStructs.h:152-160).

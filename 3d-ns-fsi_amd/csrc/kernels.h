@@ -59,9 +59,11 @@ void launchGhostGeometry(hipStream_t s, const Layout& L, const uint8_t* cls, con
 bool edgeFacesIndependent(const Layout& L, const EdgeFace& x, const EdgeFace& y);
 // All six edge faces in application order, independent faces sharing one launch (same result as
 // launchEdgeFace face by face).
+// prim != null (the stored-primitive form): the faces also write the primitive and transport values the
+// reference stores with each node, and symmetry faces mirror the stored primitives.
 template <class T>
 void launchEdgeFaces(hipStream_t s, const Layout& L, const Params& P, const EdgeFace faces[6], T* S, const double* tdev,
-                     const double* dtdev, double stageFrac);
+                     const double* dtdev, double stageFrac, T* prim = nullptr);
 template <class T>
 void launchEdgeFace(hipStream_t s, const Layout& L, const Params& P, const EdgeFace& f, T* S, const double* tdev,
                     const double* dtdev, double stageFrac);
@@ -70,7 +72,9 @@ void launchFilterCompute(hipStream_t s, const Layout& L, const Window& W, const 
                          T* tmp, T* rhoPre, T* rhoEPre);
 template <class T>
 void launchFilterWrite(hipStream_t s, const Layout& L, const int64_t* fnodes, int64_t n, T* S, const T* tmp);
+// prim != null: also the ghosts' stored primitive and transport values, derived from their new state.
 template <class T>
-void launchGhostApply(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, T* S);
+void launchGhostApply(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, T* S,
+                      T* prim = nullptr);
 
 }  // namespace nsfsi

@@ -307,9 +307,24 @@ __device__ __forceinline__ Fields<T> fields(const Layout& L, T* S) {
 // face needs no second pass.
 // Node (a, b) of the face's owned box (a along the first in-plane axis, b along the second); ts is the
 // time of the stage the boundary values belong to.
+// Stored primitive and transport values of node o (7 fields: u, v, w, p', T', mu, kappa).
+template <class T>
+__device__ __forceinline__ void storePrims(const Layout& L, T* prim, size_t o, T u, T v, T w, T p, T Tt, T mu, T ka) {
+    prim[o] = u;
+    prim[o + L.fstride] = v;
+    prim[o + 2 * L.fstride] = w;
+    prim[o + 3 * L.fstride] = p;
+    prim[o + 4 * L.fstride] = Tt;
+    prim[o + 5 * L.fstride] = mu;
+    prim[o + 6 * L.fstride] = ka;
+}
+
+// prim != null: the face also writes the primitive and transport values the reference stores with the node
+// (setFlowVariablesAtNode, src/FlowVariableGroupStructs.h:189-207), and a symmetry face mirrors the stored
+// primitives of the node two in instead of deriving them from its conserved values.
 template <class T>
 __device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, const EdgeFace& f, T* S, double ts, int a,
-                                             int b) {
+                                             int b, T* prim = nullptr) {
     // in-plane axes in the reference's loop order: x-normal (j,k), y-normal (i,k), z-normal (i,j)
     int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
     int lo[3] = {f.i0, f.j0, f.k0}, hi[3] = {f.i1, f.j1, f.k1};
@@ -351,6 +366,11 @@ __device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, c
         T rho, ru, rv, rw, rE;
         deriveCons(e, uu, vv, ww, p, Tt, rho, ru, rv, rw, rE);
         F.f[0][o] = rho; F.f[1][o] = ru; F.f[2][o] = rv; F.f[3][o] = rw; F.f[4][o] = rE;
+        if (prim) {
+            T mu, ka;
+            deriveTrans(e, Tt, mu, ka);
+            storePrims(L, prim, o, uu, vv, ww, p, Tt, mu, ka);
+        }
     } else if (f.type == NS_BC_OUTLET) {
         T a[4], b[4], v[5];
         for (int q = 0; q < 4; ++q) {
@@ -359,29 +379,53 @@ __device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, c
         }
         outletValues(e.gm1, a, b, v);
         for (int q = 0; q < 5; ++q) F.f[q][o] = v[q];
+        if (prim) {  // src/Boundary.cpp:318-331
+            const T r = v[0], p = 0;
+            const T u = v[1] / ((T)1 + r), vv = v[2] / ((T)1 + r), w = v[3] / ((T)1 + r);
+            const T Tt = (e.gamma * p - r) / ((T)1 + r);
+            T mu, ka;
+            deriveTrans(e, Tt, mu, ka);
+            storePrims(L, prim, o, u, vv, w, p, Tt, mu, ka);
+        }
     } else if (f.type == NS_BC_PERIODIC) {
         int cc[3] = {c[0], c[1], c[2]};
         int Nn = f.axis == 0 ? L.NI : (f.axis == 1 ? L.NJ : L.NK);
         cc[f.axis] = f.side == 0 ? Nn - 2 : 1;
         size_t src = L.off(cc[0], cc[1], cc[2]);
         for (int q = 0; q < 5; ++q) F.f[q][o] = F.f[q][src];
+        if (prim)
+            for (int q = 0; q < 7; ++q) prim[o + q * L.fstride] = prim[src + q * L.fstride];
     } else {  // symmetry: mirror the primitives of the node two in, negate the normal velocity
         size_t a2 = o + s2;
         T u, v, w, p, Tt, mu;
-        derivePrims(e, F.f[0][a2], F.f[1][a2], F.f[2][a2], F.f[3][a2], F.f[4][a2], u, v, w, p, Tt, mu);
+        if (prim) {
+            u = prim[a2];
+            v = prim[a2 + L.fstride];
+            w = prim[a2 + 2 * L.fstride];
+            p = prim[a2 + 3 * L.fstride];
+            Tt = prim[a2 + 4 * L.fstride];
+        } else {
+            derivePrims(e, F.f[0][a2], F.f[1][a2], F.f[2][a2], F.f[3][a2], F.f[4][a2], u, v, w, p, Tt, mu);
+        }
         u = f.axis == 0 ? -u : u;  // selects, see the inlet above
         v = f.axis == 1 ? -v : v;
         w = f.axis == 2 ? -w : w;
         T rho, ru, rv, rw, rE;
         deriveCons(e, u, v, w, p, Tt, rho, ru, rv, rw, rE);
         F.f[0][o] = rho; F.f[1][o] = ru; F.f[2][o] = rv; F.f[3][o] = rw; F.f[4][o] = rE;
+        if (prim) {
+            T ka;
+            deriveTrans(e, Tt, mu, ka);
+            storePrims(L, prim, o, u, v, w, p, Tt, mu, ka);
+        }
     }
 }
 
 template <class T>
-__global__ void k_edge_face(Layout L, Params P, EdgeFace f, T* S, const double* tdev, const double* dtdev, double frac) {
+__global__ void k_edge_face(Layout L, Params P, EdgeFace f, T* S, const double* tdev, const double* dtdev, double frac,
+                            T* prim) {
     const double ts = *tdev + frac * *dtdev;
-    edgeFaceNode(L, P, f, S, ts, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+    edgeFaceNode(L, P, f, S, ts, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, prim);
 }
 
 // Several edge faces in one launch (faces that neither write what another of them reads): face q takes
@@ -399,7 +443,7 @@ struct EdgeFaceGroup {
 
 template <class T>
 __global__ void k_edge_faces(Layout L, Params P, EdgeFaceGroup g, T* S, const double* tdev, const double* dtdev,
-                             double frac) {
+                             double frac, T* prim) {
     const int b = blockIdx.x;
     int q = 0;
     while (q + 1 < g.n && b >= g.start[q + 1]) ++q;
@@ -417,9 +461,9 @@ __global__ void k_edge_faces(Layout L, Params P, EdgeFaceGroup g, T* S, const do
             for (int c = 0; c < 5; ++c) S[o + c * L.fstride] = (T)0;
             return;
         }
-        edgeFaceNode(L, P, f, S, ts, i, bb);
+        edgeFaceNode(L, P, f, S, ts, i, bb, prim);
     } else {
-        edgeFaceNode(L, P, f, S, ts, a, bb);
+        edgeFaceNode(L, P, f, S, ts, a, bb, prim);
     }
 }
 
@@ -561,6 +605,20 @@ __device__ __forceinline__ void ghostApplyField(const Layout& L, const Params& P
     Sq[(size_t)G.node[g]] = (T)(q >= 1 && q <= 3 ? -ipv : ipv);
 }
 
+// Stored primitive and transport values of the ghosts after their update (src/Boundary.cpp:447-455:
+// derivePrimitiveVariables and deriveTransportProperties of the ghost's conserved values).
+template <class T>
+__global__ void k_ghost_prims(Layout L, Params P, GhostTables G, int64_t n, const T* S, T* prim) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const size_t o = (size_t)G.node[g];
+    Eos<T> e = Eos<T>::make(P);
+    T u, v, w, p, Tt, mu;
+    derivePrims(e, S[o], S[o + L.fstride], S[o + 2 * L.fstride], S[o + 3 * L.fstride], S[o + 4 * L.fstride], u, v, w, p,
+                Tt, mu);
+    storePrims(L, prim, o, u, v, w, p, Tt, mu, mu / e.gm1Pr);
+}
+
 template <class T>
 __global__ void k_ghost_apply(Layout L, Params P, GhostTables G, int64_t n, T* S) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -640,7 +698,7 @@ void launchEdgeFace(hipStream_t s, const Layout& L, const Params& P, const EdgeF
     int lo[3] = {f.i0, f.j0, f.k0}, hi[3] = {f.i1, f.j1, f.k1};
     int n1 = hi[ax1] - lo[ax1] + 1, n2 = hi[ax2] - lo[ax2] + 1;
     if (n1 <= 0 || n2 <= 0) return;
-    hipLaunchKernelGGL(k_edge_face<T>, dim3(blocks(n1, 64), n2), dim3(64), 0, s, L, P, f, S, tdev, dtdev, frac);
+    hipLaunchKernelGGL(k_edge_face<T>, dim3(blocks(n1, 64), n2), dim3(64), 0, s, L, P, f, S, tdev, dtdev, frac, (T*)nullptr);
 }
 
 namespace {
@@ -683,13 +741,13 @@ bool edgeFacesIndependent(const Layout& L, const EdgeFace& x, const EdgeFace& y)
 
 template <class T>
 void launchEdgeFaces(hipStream_t s, const Layout& L, const Params& P, const EdgeFace faces[6], T* S, const double* tdev,
-                     const double* dtdev, double frac) {
+                     const double* dtdev, double frac, T* prim) {
     // faces in application order, grouped greedily: a face joins the current launch when it is independent
     // of every face already in it; the groups run in order, so the result equals face-by-face application
     EdgeFaceGroup g{};
     auto flush = [&]() {
         if (g.n > 0 && g.start[g.n] > 0)
-            hipLaunchKernelGGL(k_edge_faces<T>, dim3(g.start[g.n]), dim3(64), 0, s, L, P, g, S, tdev, dtdev, frac);
+            hipLaunchKernelGGL(k_edge_faces<T>, dim3(g.start[g.n]), dim3(64), 0, s, L, P, g, S, tdev, dtdev, frac, prim);
         g = EdgeFaceGroup{};
     };
     for (int q = 0; q < 6; ++q) {
@@ -725,9 +783,10 @@ void launchFilterWrite(hipStream_t s, const Layout& L, const int64_t* fn, int64_
 }
 
 template <class T>
-void launchGhostApply(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, T* S) {
+void launchGhostApply(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, T* S, T* prim) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_ghost_apply<T>, dim3(blocks(5 * n, 64)), dim3(64), 0, s, L, P, G, n, S);
+    if (prim) hipLaunchKernelGGL(k_ghost_prims<T>, dim3(blocks(n, 64)), dim3(64), 0, s, L, P, G, n, (const T*)S, prim);
 }
 
 template <class T>
@@ -768,13 +827,13 @@ void launchGhostApplyBatched(hipStream_t s, const CaseView<T>* cv, const int64_t
     template void launchFilterWriteBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);      \
     template void launchGhostApplyBatched<T>(hipStream_t, const CaseView<T>*, const int64_t*, int64_t, int, int, int);       \
     template void launchEdgeFaces<T>(hipStream_t, const Layout&, const Params&, const EdgeFace[6], T*, const double*,         \
-                                     const double*, double);                                                                \
+                                     const double*, double, T*);                                                            \
     template void launchEdgeFace<T>(hipStream_t, const Layout&, const Params&, const EdgeFace&, T*, const double*,            \
                                     const double*, double);                                                                   \
     template void launchFilterCompute<T>(hipStream_t, const Layout&, const Window&, const int64_t*, int64_t, const T*, T*,   \
                                          T*, T*);                                                                             \
     template void launchFilterWrite<T>(hipStream_t, const Layout&, const int64_t*, int64_t, T*, const T*);                  \
-    template void launchGhostApply<T>(hipStream_t, const Layout&, const Params&, const GhostTables&, int64_t, T*);
+    template void launchGhostApply<T>(hipStream_t, const Layout&, const Params&, const GhostTables&, int64_t, T*, T*);
 NS_INST(double)
 NS_INST(float)
 #undef NS_INST

@@ -47,9 +47,11 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
     return r;
 }
 
+// prim != null (stored primitives): every node's u, v, w, p' and mu come from the stored fields, as
+// Solver::getInviscidTimeStepLimit / getViscousTimeStepLimit read them (src/Solver.cpp:91-126).
 template <class T>
 __global__ void k_step_reduce(Layout L, Params P, const T* U, const T* A, const uint8_t* cls, Window W, const T* rhoPre,
-                              const T* rhoEPre, int withNorms, double* partials) {
+                              const T* rhoEPre, int withNorms, double* partials, const T* prim) {
     // one block per group of x-rows: no per-node index division
     __shared__ double sh[8];
     const size_t F = L.fstride;
@@ -72,6 +74,14 @@ __global__ void k_step_reduce(Layout L, Params P, const T* U, const T* A, const 
                     double d = (double)U[o + q * F] - (double)a[q];
                     sm[q] += d * d;
                 }
+            if (prim) {
+                const double cs = sqrt((1 + P.gamma * (double)prim[o + 3 * F]) / (1 + (double)a[0]));
+                mx[0] = fmax(mx[0], cs + fabs((double)prim[o]));
+                mx[1] = fmax(mx[1], cs + fabs((double)prim[o + F]));
+                mx[2] = fmax(mx[2], cs + fabs((double)prim[o + 2 * F]));
+                mx[3] = fmax(mx[3], (double)prim[o + 5 * F] / ((double)a[0] + 1) * vm);
+                continue;
+            }
             if (c & (CLS_DT_SKIP | CLS_INACTIVE_IC)) continue;
             T rho = a[0], rE = a[4];
             if (rowW && (c & CLS_FILTER)) {
@@ -373,9 +383,10 @@ template int launchBoundaryReduce<float>(hipStream_t, const Layout&, const Param
 
 template <class T>
 void launchStepReduce(hipStream_t s, const Layout& L, const Params& P, const T* U, const T* A, const uint8_t* cls,
-                      const Window& W, const T* rhoPre, const T* rhoEPre, int withNorms, double* partials, int nBlocks) {
+                      const Window& W, const T* rhoPre, const T* rhoEPre, int withNorms, double* partials, int nBlocks,
+                      const T* prim) {
     hipLaunchKernelGGL(k_step_reduce<T>, dim3(nBlocks), dim3(RB), 0, s, L, P, U, A, cls, W, rhoPre, rhoEPre, withNorms,
-                       partials);
+                       partials, prim);
 }
 
 template <class T>
@@ -397,9 +408,11 @@ int reducePartialSlots() { return NP; }
 int inactiveReduceBlocks(int64_t n) { return (int)std::min<int64_t>(64, std::max<int64_t>(1, (n + RB * 8 - 1) / (RB * 8))); }
 
 template void launchStepReduce<double>(hipStream_t, const Layout&, const Params&, const double*, const double*,
-                                       const uint8_t*, const Window&, const double*, const double*, int, double*, int);
+                                       const uint8_t*, const Window&, const double*, const double*, int, double*, int,
+                                       const double*);
 template void launchStepReduce<float>(hipStream_t, const Layout&, const Params&, const float*, const float*,
-                                      const uint8_t*, const Window&, const float*, const float*, int, double*, int);
+                                      const uint8_t*, const Window&, const float*, const float*, int, double*, int,
+                                      const float*);
 template void launchInactiveReduce<double>(hipStream_t, const Layout&, const Params&, const double*, const int64_t*,
                                            const double*, int64_t, double*);
 template void launchInactiveReduce<float>(hipStream_t, const Layout&, const Params&, const float*, const int64_t*,

@@ -404,18 +404,32 @@ void Case::uploadState(const std::vector<T>& h) {
     if (nInactive > 0)
         hipLaunchKernelGGL(k_gather_ic<T>, dim3(nblk(nInactive, 128)), dim3(128), 0, stream, L, P, (const T*)U,
                            inactiveList, nInactive, inactiveIC);
+    // stored primitive and transport fields of the initial state (FP64; deriveTransportProperties' conductivity)
+    if (!fp32) {
+        launchDerive<T>(stream, L, P, (const T*)U, (T*)prim, cls, W, (const T*)rhoPre, (const T*)rhoEPre, 2);
+        primsStored = true;
+    }
     // updateTimeStepSize(0) on the initial state
     computeDt<T>(0);
     hipCheck(hipStreamSynchronize(stream), "sync");
     hipCheck(hipGetLastError(), "init kernels");
 }
 
+void Case::syncStoredPrims() {
+    if (!literal() || primsStored) return;
+    launchDerive<double>(stream, L, P, (const double*)U, (double*)prim, cls, W, (const double*)rhoPre,
+                         (const double*)rhoEPre, 2);
+    primsStored = true;
+}
+
 template <class T>
 void Case::computeDt(int mode) {
+    // the stored-primitive form scans the stored fields at every node (untouched solid nodes included)
+    const bool stored = !fp32 && primsStored && (literal() || mode == 0);
     launchStepReduce<T>(stream, L, P, (const T*)U, mode == 0 ? (const T*)U : (const T*)A, cls, W, (const T*)rhoPre,
-                        (const T*)rhoEPre, mode, partials, nPartials);
+                        (const T*)rhoEPre, mode, partials, nPartials, stored ? (const T*)prim : nullptr);
     int npart = nPartials;
-    if (nInactive > 0) {
+    if (nInactive > 0 && !stored) {
         launchInactiveReduce<T>(stream, L, P, mode == 0 ? (const T*)U : (const T*)A, inactiveList, inactiveIC, nInactive,
                                 partials + (size_t)nPartials * reducePartialSlots());
         npart += inactiveReduceBlocks(nInactive);
@@ -425,16 +439,16 @@ void Case::computeDt(int mode) {
 }
 
 template <class T>
-void Case::applyBCs(T* S, double frac, bool outletDone) {
+void Case::applyBCs(T* S, double frac, bool outletDone, T* storedPrims) {
     // Mesh::applyAllBoundaryConditions, src/Mesh.cpp:155-167: edge faces in reverse claim order, then the
     // immersed body (filter first, then ghosts; src/Boundary.cpp:431-456)
     const double* tdev = &scal->t;
     const double* dtdev = &scal->dt;
-    launchEdgeFaces<T>(stream, L, P, outletDone ? bcFaces : faces, S, tdev, dtdev, frac);
+    launchEdgeFaces<T>(stream, L, P, outletDone ? bcFaces : faces, S, tdev, dtdev, frac, storedPrims);
     if (B.type == NS_BODY_NONE) return;
     launchFilterCompute<T>(stream, L, W, filterList, nFilter, S, (T*)ftmp, (T*)rhoPre, (T*)rhoEPre);
     launchFilterWrite<T>(stream, L, filterList, nFilter, S, (const T*)ftmp);
-    launchGhostApply<T>(stream, L, P, G, nGhost, S);
+    launchGhostApply<T>(stream, L, P, G, nGhost, S, storedPrims);
 }
 
 // Solver::marchTimeStep, src/Solver.cpp:130-167 (without the rho filter, see ns_init)
@@ -472,6 +486,7 @@ void Case::stepOnce() {
         hipCheck(hipMemcpyAsync(tapeNow->PE[slot], rhoEPre, wb, hipMemcpyDeviceToDevice, stream), "tape");
     };
     tapeCopy(0, u);
+    if (!literal()) primsStored = false;  // the other forms do not keep the stored primitive fields
     for (int st = 1; st <= 4; ++st) {
         if (stageKernel == 1) {
             mark(0, true);
@@ -480,6 +495,13 @@ void Case::stepOnce() {
                                                   fusedScans ? partials : nullptr, cfg.conv_history > 0,
                                                   rhoFilter ? fRho : nullptr, rhoFilter ? fFlag : nullptr, comp,
                                                   stageOutlet());
+            mark(0, false);
+        } else if (literal()) {
+            // stored primitives: the slopes read the stored fields; after the combine the
+            // interior primitives are updated (src/Solver.cpp:446-479) and the boundary conditions write theirs
+            mark(0, true);
+            launchRhsPlain<T>(stream, st, L, P, in[st - 1], pr, u, out[st - 1], a, cls, dtdev, rhoFilter ? fRho : nullptr,
+                              rhoFilter ? fFlag : nullptr, nullptr, true);
             mark(0, false);
         } else {
             mark(1, true);
@@ -491,8 +513,13 @@ void Case::stepOnce() {
             mark(0, false);
         }
         if (st == 1 && rhoFilter) launchRhoCommit<T>(stream, L, u, fRho, a, cls, fFlag);
+        if (literal()) {
+            mark(1, true);
+            launchDerive<T>(stream, L, P, out[st - 1], pr, cls, W, (const T*)rhoPre, (const T*)rhoEPre, 1);
+            mark(1, false);
+        }
         mark(2, true);
-        applyBCs<T>(out[st - 1], frac[st - 1], stageKernel == 1 && stageOutlet());
+        applyBCs<T>(out[st - 1], frac[st - 1], stageKernel == 1 && stageOutlet(), literal() ? pr : nullptr);
         mark(2, false);
         if (st < 4) tapeCopy(st, out[st - 1]);
     }
@@ -627,6 +654,7 @@ void Case::mark(int group, bool begin) {
 void Case::profile(int nsteps, double ms[4], int launches[4]) {
     if (!initialized) throw StateError("profile before init");
     checkSchedule(nsteps);
+    syncStoredPrims();
     double sum[4] = {0, 0, 0, 0};
     int cnt[4] = {0, 0, 0, 0};
     auto collect = [&](bool keep) {
@@ -783,6 +811,7 @@ void Case::checkSchedule(int64_t nsteps) const {
 void Case::step(int nsteps) {
     if (!initialized) throw StateError("step before init");
     checkSchedule(nsteps);
+    syncStoredPrims();
     int done = 0;
     while (done < nsteps) {
         int batch = std::min(nsteps - done, histCap);
@@ -822,6 +851,7 @@ void Case::stepTaped(StageTape* tape) {
     if (!initialized) throw StateError("step before init");
     if (fp32) throw Unsupported("the adjoint runs on the FP64 form");
     checkSchedule(1);
+    syncStoredPrims();
     tapeNow = tape;
     stepOnce<double>();  // swaps U and A itself
     tapeNow = nullptr;
@@ -858,6 +888,11 @@ void Case::getField(int which, double* out) {
     };
     hipCheck(hipStreamSynchronize(stream), "sync");
     if (which < 5) { copyOut(U, which); return; }
+    if (literal() && primsStored) {  // the stored fields are the reference's values
+        copyOut(prim, which - 5);
+        return;
+    }
+    primsStored = false;  // prim is rewritten below
     // primitive and transport values as the reference keeps them: derived from the state, pre-filter
     // values at filter nodes, initial values at untouched solid nodes
     if (fp32) {

@@ -31,13 +31,17 @@ struct FilterSchedule {
 };
 
 // ---- unfused stage (stage_plain.hip)
+// Primitive and transport fields (7: u, v, w, p', T', mu, kappa) from a state. mode 0: every node, pre-filter
+// density and energy at filter nodes; mode 1 (stored primitives, the reference's updatePrimitiveVariables):
+// FluidInterior nodes only; mode 2: every node with the boundary conditions' conductivity mu / ((gamma-1) Pr).
 template <class T>
 void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T* prim, const uint8_t* cls, const Window& W,
-                  const T* rhoPre, const T* rhoEPre);
+                  const T* rhoPre, const T* rhoEPre, int mode = 0);
+// literal (FP64 only): the reference's own combine (slope sums in A, final U + dt/6 * sum), see stage_plain.hip.
 template <class T>
 void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                     T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* filteredRho,
-                    const int32_t* filterFlag, int8_t* comp);
+                    const int32_t* filterFlag, int8_t* comp, bool literal = false);
 
 // Periodic density filter (Mesh::applyFilter_ifAppropriate, src/Mesh.cpp:108-140), in three parts:
 // the flag for this step from (t, timeLevel); the filtered interior density of U into `out`; after the
@@ -116,8 +120,10 @@ int launchBoundaryReduce(hipStream_t s, const Layout& L, const Params& P, const 
 
 // ---- end-of-step reductions (reduce.hip)
 template <class T>
+// prim != null: the dt maxima read the stored primitive fields at every node (the stored-primitive form).
 void launchStepReduce(hipStream_t s, const Layout& L, const Params& P, const T* U, const T* A, const uint8_t* cls,
-                      const Window& W, const T* rhoPre, const T* rhoEPre, int withNorms, double* partials, int nBlocks);
+                      const Window& W, const T* rhoPre, const T* rhoEPre, int withNorms, double* partials, int nBlocks,
+                      const T* prim = nullptr);
 template <class T>
 void launchInactiveReduce(hipStream_t s, const Layout& L, const Params& P, const T* A, const int64_t* nodes,
                           const double* icPrim, int64_t n, double* partial);

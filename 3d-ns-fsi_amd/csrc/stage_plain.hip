@@ -46,21 +46,27 @@ __device__ __forceinline__ SegSlot segSlot(const Layout& L, int j, int k) {
     return r;
 }
 
-template <class T>
+// MODE 0: every node from S, pre-filter density and energy at filter nodes (the primitives of the stage input
+// as the reference holds them there). MODE 1 (stored primitives): updatePrimitiveVariables, src/Solver.cpp:
+// 446-479 — FluidInterior nodes only, from S as it leaves the stage combine; the other nodes keep what the
+// boundary conditions wrote. MODE 2: every node from S with deriveTransportProperties' conductivity (the
+// initial condition, src/Solver.cpp:31-75, and a state set from outside).
+template <class T, int MODE>
 __global__ void k_derive(Layout L, Params P, const T* S, T* prim, const uint8_t* cls, Window W, const T* rhoPre,
                          const T* rhoEPre) {
     const int j = blockIdx.y, k = blockIdx.z;
     const SegSlot sl = segSlot(L, j, k);
     if (sl.pad) {
-        if (sl.write)
+        if (sl.write && MODE != 1)
             for (int q = 0; q < 7; ++q) prim[sl.o + q * L.fstride] = (T)0;
         return;
     }
     const int i = sl.i;
     const size_t o = sl.o;
+    if (MODE == 1 && (cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) return;
     Eos<T> e = Eos<T>::make(P);
     T rho = S[o], rE = S[o + 4 * L.fstride];
-    if (cls[o] & CLS_FILTER) {
+    if (MODE == 0 && (cls[o] & CLS_FILTER)) {
         size_t w = W.idx(i, j, k);
         rho = rhoPre[w];
         rE = rhoEPre[w];
@@ -73,10 +79,13 @@ __global__ void k_derive(Layout L, Params P, const T* S, T* prim, const uint8_t*
     prim[o + 3 * L.fstride] = p;
     prim[o + 4 * L.fstride] = Tt;
     prim[o + 5 * L.fstride] = mu;
-    prim[o + 6 * L.fstride] = mu * e.pf;
+    prim[o + 6 * L.fstride] = MODE == 2 ? mu / e.gm1Pr : mu * e.pf;
 }
 
-template <class T, int STAGE, bool KAHAN>
+// LIT (FP64): the reference's own combine. A holds the slope sum k1, k1 + 2 k2, k1 + 2 k2 + 2 k3 at interior
+// nodes through stages 1-3 and stage 4 forms U + dt/6 (that sum + k4) (compute_RK4_final_step,
+// src/Solver.cpp:494-507), the same operations in the same order as the reference's four stored slopes.
+template <class T, int STAGE, bool KAHAN, bool LIT>
 __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, const T* U, T* Sout, T* A, const uint8_t* cls,
                             const double* dtdev, const T* fRho, const int32_t* fFlag, int8_t* comp) {
     const bool rhoFiltered = STAGE == 1 && fFlag && *fFlag;
@@ -106,9 +115,12 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
     const T n2dx = (T)-1 / ((T)2 * dx), n2dy = (T)-1 / ((T)2 * dy), n2dz = (T)-1 / ((T)2 * dz);
     const T p23xx = (T)2 / ((T)3 * dx * dx), p23yy = (T)2 / ((T)3 * dy * dy), p23zz = (T)2 / ((T)3 * dz * dz);
     const T p12xx = (T)1 / ((T)2 * dx * dx), p12yy = (T)1 / ((T)2 * dy * dy), p12zz = (T)1 / ((T)2 * dz * dz);
-    const T c6xy = (T)-1 / ((T)6 * dx * dy), c4xy = (T)1 / ((T)4 * dx * dy);
-    const T c6xz = (T)-1 / ((T)6 * dx * dz), c4xz = (T)1 / ((T)4 * dx * dz);
-    const T c6yz = (T)-1 / ((T)6 * dy * dz), c4yz = (T)1 / ((T)4 * dy * dz);
+    // mixed-derivative factors with the reference's operand order per use (src/Solver.cpp:202-211, 249-258,
+    // 296-305, 345-359): 1/(6 dy dx) and 1/(6 dx dy) can round differently
+    const T n6ydx = (T)-1 / ((T)6 * dy * dx), n6zdx = (T)-1 / ((T)6 * dz * dx), n6xdy = (T)-1 / ((T)6 * dx * dy);
+    const T n6zdy = (T)-1 / ((T)6 * dz * dy), n6xdz = (T)-1 / ((T)6 * dx * dz), n6ydz = (T)-1 / ((T)6 * dy * dz);
+    const T p4xdy = (T)1 / ((T)4 * dx * dy), p4xdz = (T)1 / ((T)4 * dx * dz), p4ydx = (T)1 / ((T)4 * dy * dx);
+    const T p4ydz = (T)1 / ((T)4 * dy * dz), p4zdx = (T)1 / ((T)4 * dz * dx), p4zdy = (T)1 / ((T)4 * dz * dy);
     const T H0 = (T)(1 / (P.gamma - 1));
     const T* ru = Sin + F;
     const T* rv = Sin + 2 * F;
@@ -135,10 +147,10 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         T vXX = p23xx * ((mu[e_] + mu[o]) * (u[e_] - u[o]) - (mu[w_] + mu[o]) * (u[o] - u[w_]));
         T vYY = p12yy * ((mu[n_] + mu[o]) * (u[n_] - u[o]) - (mu[s_] + mu[o]) * (u[o] - u[s_]));
         T vZZ = p12zz * ((mu[u_] + mu[o]) * (u[u_] - u[o]) - (mu[d_] + mu[o]) * (u[o] - u[d_]));
-        T vXY = c4xy * (mu[n_] * (v[n_ + XP] - v[n_ + XM]) - mu[s_] * (v[s_ + XP] - v[s_ + XM]));
-        T vYX = c6xy * (mu[e_] * (v[e_ + Y] - v[e_ - Y]) - mu[w_] * (v[w_ + Y] - v[w_ - Y]));
-        T vXZ = c4xz * (mu[u_] * (w[u_ + XP] - w[u_ + XM]) - mu[d_] * (w[d_ + XP] - w[d_ + XM]));
-        T vZX = c6xz * (mu[e_] * (w[e_ + Z] - w[e_ - Z]) - mu[w_] * (w[w_ + Z] - w[w_ - Z]));
+        T vXY = p4xdy * (mu[n_] * (v[n_ + XP] - v[n_ + XM]) - mu[s_] * (v[s_ + XP] - v[s_ + XM]));
+        T vYX = n6ydx * (mu[e_] * (v[e_ + Y] - v[e_ - Y]) - mu[w_] * (v[w_ + Y] - v[w_ - Y]));
+        T vXZ = p4xdz * (mu[u_] * (w[u_ + XP] - w[u_ + XM]) - mu[d_] * (w[d_ + XP] - w[d_ + XM]));
+        T vZX = n6zdx * (mu[e_] * (w[e_ + Z] - w[e_ - Z]) - mu[w_] * (w[w_ + Z] - w[w_ - Z]));
         kq[1] = cX + cY + cZ + vXX + vYY + vZZ + vXY + vYX + vXZ + vZX;
     }
     {
@@ -148,10 +160,10 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         T vXX = p12xx * ((mu[e_] + mu[o]) * (v[e_] - v[o]) - (mu[w_] + mu[o]) * (v[o] - v[w_]));
         T vYY = p23yy * ((mu[n_] + mu[o]) * (v[n_] - v[o]) - (mu[s_] + mu[o]) * (v[o] - v[s_]));
         T vZZ = p12zz * ((mu[u_] + mu[o]) * (v[u_] - v[o]) - (mu[d_] + mu[o]) * (v[o] - v[d_]));
-        T vYX = c4xy * (mu[e_] * (u[e_ + Y] - u[e_ - Y]) - mu[w_] * (u[w_ + Y] - u[w_ - Y]));
-        T vXY = c6xy * (mu[n_] * (u[n_ + XP] - u[n_ + XM]) - mu[s_] * (u[s_ + XP] - u[s_ + XM]));
-        T vYZ = c4yz * (mu[u_] * (w[u_ + Y] - w[u_ - Y]) - mu[d_] * (w[d_ + Y] - w[d_ - Y]));
-        T vZY = c6yz * (mu[n_] * (w[n_ + Z] - w[n_ - Z]) - mu[s_] * (w[s_ + Z] - w[s_ - Z]));
+        T vYX = p4ydx * (mu[e_] * (u[e_ + Y] - u[e_ - Y]) - mu[w_] * (u[w_ + Y] - u[w_ - Y]));
+        T vXY = n6xdy * (mu[n_] * (u[n_ + XP] - u[n_ + XM]) - mu[s_] * (u[s_ + XP] - u[s_ + XM]));
+        T vYZ = p4ydz * (mu[u_] * (w[u_ + Y] - w[u_ - Y]) - mu[d_] * (w[d_ + Y] - w[d_ - Y]));
+        T vZY = n6zdy * (mu[n_] * (w[n_ + Z] - w[n_ - Z]) - mu[s_] * (w[s_ + Z] - w[s_ - Z]));
         kq[2] = cX + cY + cZ + vXX + vYY + vZZ + vYX + vXY + vYZ + vZY;
     }
     {
@@ -161,10 +173,10 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         T vXX = p12xx * ((mu[e_] + mu[o]) * (w[e_] - w[o]) - (mu[w_] + mu[o]) * (w[o] - w[w_]));
         T vYY = p12yy * ((mu[n_] + mu[o]) * (w[n_] - w[o]) - (mu[s_] + mu[o]) * (w[o] - w[s_]));
         T vZZ = p23zz * ((mu[u_] + mu[o]) * (w[u_] - w[o]) - (mu[d_] + mu[o]) * (w[o] - w[d_]));
-        T vZX = c4xz * (mu[e_] * (u[e_ + Z] - u[e_ - Z]) - mu[w_] * (u[w_ + Z] - u[w_ - Z]));
-        T vXZ = c6xz * (mu[u_] * (u[u_ + XP] - u[u_ + XM]) - mu[d_] * (u[d_ + XP] - u[d_ + XM]));
-        T vZY = c4yz * (mu[n_] * (v[n_ + Z] - v[n_ - Z]) - mu[s_] * (v[s_ + Z] - v[s_ - Z]));
-        T vYZ = c6yz * (mu[u_] * (v[u_ + Y] - v[u_ - Y]) - mu[d_] * (v[d_ + Y] - v[d_ - Y]));
+        T vZX = p4zdx * (mu[e_] * (u[e_ + Z] - u[e_ - Z]) - mu[w_] * (u[w_ + Z] - u[w_ - Z]));
+        T vXZ = n6xdz * (mu[u_] * (u[u_ + XP] - u[u_ + XM]) - mu[d_] * (u[d_ + XP] - u[d_ + XM]));
+        T vZY = p4zdy * (mu[n_] * (v[n_ + Z] - v[n_ - Z]) - mu[s_] * (v[s_ + Z] - v[s_ - Z]));
+        T vYZ = n6ydz * (mu[u_] * (v[u_ + Y] - v[u_ - Y]) - mu[d_] * (v[d_ + Y] - v[d_ - Y]));
         kq[3] = cX + cY + cZ + vXX + vYY + vZZ + vZX + vXZ + vZY + vYZ;
     }
     // energy (src/Solver.cpp:378-416)
@@ -180,24 +192,24 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         T vx = p23xx * ((mE * uE + mP * uP) * (uE - uP) - (mW * uW + mP * uP) * (uP - uW))
              + p12xx * ((mE * vE + mP * vP) * (vE - vP) - (mW * vW + mP * vP) * (vP - vW))
              + p12xx * ((mE * wE + mP * wP) * (wE - wP) - (mW * wW + mP * wP) * (wP - wW))
-             + c6xy * (mE * uE * (v[e_ + Y] - v[e_ - Y]) - mW * uW * (v[w_ + Y] - v[w_ - Y]))
-             + c6xz * (mE * uE * (w[e_ + Z] - w[e_ - Z]) - mW * uW * (w[w_ + Z] - w[w_ - Z]))
-             + c4xy * (mE * vE * (u[e_ + Y] - u[e_ - Y]) - mW * vW * (u[w_ + Y] - u[w_ - Y]))
-             + c4xz * (mE * wE * (u[e_ + Z] - u[e_ - Z]) - mW * wW * (u[w_ + Z] - u[w_ - Z]));
+             + n6ydx * (mE * uE * (v[e_ + Y] - v[e_ - Y]) - mW * uW * (v[w_ + Y] - v[w_ - Y]))
+             + n6zdx * (mE * uE * (w[e_ + Z] - w[e_ - Z]) - mW * uW * (w[w_ + Z] - w[w_ - Z]))
+             + p4ydx * (mE * vE * (u[e_ + Y] - u[e_ - Y]) - mW * vW * (u[w_ + Y] - u[w_ - Y]))
+             + p4zdx * (mE * wE * (u[e_ + Z] - u[e_ - Z]) - mW * wW * (u[w_ + Z] - u[w_ - Z]));
         T vy = p12yy * ((mN * uN + mP * uP) * (uN - uP) - (mS * uS + mP * uP) * (uP - uS))
              + p23yy * ((mN * vN + mP * vP) * (vN - vP) - (mS * vS + mP * vP) * (vP - vS))
              + p12yy * ((mN * wN + mP * wP) * (wN - wP) - (mS * wS + mP * wP) * (wP - wS))
-             + c6xy * (mN * vN * (u[n_ + XP] - u[n_ + XM]) - mS * vS * (u[s_ + XP] - u[s_ + XM]))
-             + c6yz * (mN * vN * (w[n_ + Z] - w[n_ - Z]) - mS * vS * (w[s_ + Z] - w[s_ - Z]))
-             + c4xy * (mN * uN * (v[n_ + XP] - v[n_ + XM]) - mS * uS * (v[s_ + XP] - v[s_ + XM]))
-             + c4yz * (mN * wN * (v[n_ + Z] - v[n_ - Z]) - mS * wS * (v[s_ + Z] - v[s_ - Z]));
+             + n6ydx * (mN * vN * (u[n_ + XP] - u[n_ + XM]) - mS * vS * (u[s_ + XP] - u[s_ + XM]))
+             + n6zdy * (mN * vN * (w[n_ + Z] - w[n_ - Z]) - mS * vS * (w[s_ + Z] - w[s_ - Z]))
+             + p4ydx * (mN * uN * (v[n_ + XP] - v[n_ + XM]) - mS * uS * (v[s_ + XP] - v[s_ + XM]))
+             + p4zdy * (mN * wN * (v[n_ + Z] - v[n_ - Z]) - mS * wS * (v[s_ + Z] - v[s_ - Z]));
         T vz = p12zz * ((mU * uU + mP * uP) * (uU - uP) - (mD * uD + mP * uP) * (uP - uD))
              + p12zz * ((mU * vU + mP * vP) * (vU - vP) - (mD * vD + mP * vP) * (vP - vD))
              + p23zz * ((mU * wU + mP * wP) * (wU - wP) - (mD * wD + mP * wP) * (wP - wD))
-             + c6xz * (mU * wU * (u[u_ + XP] - u[u_ + XM]) - mD * wD * (u[d_ + XP] - u[d_ + XM]))
-             + c6yz * (mU * wU * (v[u_ + Y] - v[u_ - Y]) - mD * wD * (v[d_ + Y] - v[d_ - Y]))
-             + c4xz * (mU * uU * (w[u_ + XP] - w[u_ + XM]) - mD * uD * (w[d_ + XP] - w[d_ + XM]))
-             + c4yz * (mU * vU * (w[u_ + Y] - w[u_ - Y]) - mD * vD * (w[d_ + Y] - w[d_ - Y]));
+             + n6zdx * (mU * wU * (u[u_ + XP] - u[u_ + XM]) - mD * wD * (u[d_ + XP] - u[d_ + XM]))
+             + n6zdy * (mU * wU * (v[u_ + Y] - v[u_ - Y]) - mD * wD * (v[d_ + Y] - v[d_ - Y]))
+             + p4zdx * (mU * uU * (w[u_ + XP] - w[u_ + XM]) - mD * uD * (w[d_ + XP] - w[d_ + XM]))
+             + p4zdy * (mU * vU * (w[u_ + Y] - w[u_ - Y]) - mD * vD * (w[d_ + Y] - w[d_ - Y]));
         T heat = p12xx * ((ka[e_] + kP) * (Tt[e_] - TP) - (ka[w_] + kP) * (TP - Tt[w_]))
                + p12yy * ((ka[n_] + kP) * (Tt[n_] - TP) - (ka[s_] + kP) * (TP - Tt[s_]))
                + p12zz * ((ka[u_] + kP) * (Tt[u_] - TP) - (ka[d_] + kP) * (TP - Tt[d_]));
@@ -209,14 +221,17 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
         for (int q = 0; q < 5; ++q) {
             T u0 = (q == 0 && rhoFiltered) ? fRho[o] : U[o + q * F];
             Sout[o + q * F] = u0 + h2 * kq[q];
-            A[o + q * F] = KAHAN ? h6 * kq[q] : u0 + h6 * kq[q];
+            A[o + q * F] = LIT ? kq[q] : (KAHAN ? h6 * kq[q] : u0 + h6 * kq[q]);
         }
     } else if (STAGE == 2 || STAGE == 3) {
         const T hs = (T)(STAGE == 2 ? dtd / 2 : dtd), h3 = (T)(dtd / 3);
         for (int q = 0; q < 5; ++q) {
             Sout[o + q * F] = U[o + q * F] + hs * kq[q];
-            A[o + q * F] = A[o + q * F] + h3 * kq[q];
+            A[o + q * F] = LIT ? A[o + q * F] + (T)2 * kq[q] : A[o + q * F] + h3 * kq[q];
         }
+    } else if (LIT) {
+        const T f = (T)(dtd / 6);
+        for (int q = 0; q < 5; ++q) A[o + q * F] = U[o + q * F] + f * (A[o + q * F] + kq[q]);
     } else {
         const T h6 = (T)(dtd / 6);
         for (int q = 0; q < 5; ++q) {
@@ -236,32 +251,34 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
 
 template <class T>
 void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T* prim, const uint8_t* cls, const Window& W,
-                  const T* rhoPre, const T* rhoEPre) {
+                  const T* rhoPre, const T* rhoEPre, int mode) {
     dim3 blk(Layout::kSegSlots), grd((L.NI + Layout::kSegNodes - 1) / Layout::kSegNodes, L.NJ, L.NK);
-    hipLaunchKernelGGL(k_derive<T>, grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
+    if (mode == 1) hipLaunchKernelGGL((k_derive<T, 1>), grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
+    else if (mode == 2) hipLaunchKernelGGL((k_derive<T, 2>), grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
+    else hipLaunchKernelGGL((k_derive<T, 0>), grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
 }
 
-template <class T, bool K>
+template <class T, bool K, bool LIT>
 void launchRhsPlainK(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                      T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag,
                      int8_t* comp) {
     dim3 blk(Layout::kSegSlots), grd((L.NI + Layout::kSegNodes - 1) / Layout::kSegNodes, L.NJ, L.NK);
     switch (stage) {
         case 1:
-            hipLaunchKernelGGL((k_rhs_plain<T, 1, K>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho,
+            hipLaunchKernelGGL((k_rhs_plain<T, 1, K, LIT>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho,
                                fFlag, comp);
             break;
         case 2:
-            hipLaunchKernelGGL((k_rhs_plain<T, 2, K>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr,
-                               nullptr, comp);
+            hipLaunchKernelGGL((k_rhs_plain<T, 2, K, LIT>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev,
+                               nullptr, nullptr, comp);
             break;
         case 3:
-            hipLaunchKernelGGL((k_rhs_plain<T, 3, K>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr,
-                               nullptr, comp);
+            hipLaunchKernelGGL((k_rhs_plain<T, 3, K, LIT>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev,
+                               nullptr, nullptr, comp);
             break;
         default:
-            hipLaunchKernelGGL((k_rhs_plain<T, 4, K>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, nullptr,
-                               nullptr, comp);
+            hipLaunchKernelGGL((k_rhs_plain<T, 4, K, LIT>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev,
+                               nullptr, nullptr, comp);
             break;
     }
 }
@@ -269,16 +286,19 @@ void launchRhsPlainK(hipStream_t s, int stage, const Layout& L, const Params& P,
 template <class T>
 void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                     T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag,
-                    int8_t* comp) {
+                    int8_t* comp, bool literal) {
     if constexpr (sizeof(T) == 4) {
+        if (literal) throw std::invalid_argument("the reference's own combine is the FP64 form");
         if (comp) {
-            launchRhsPlainK<T, true>(s, stage, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag, comp);
+            launchRhsPlainK<T, true, false>(s, stage, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag, comp);
             return;
         }
+        launchRhsPlainK<T, false, false>(s, stage, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag, nullptr);
     } else {
         if (comp) throw std::invalid_argument("compensated combine needs FP32 fields");
+        if (literal) launchRhsPlainK<T, false, true>(s, stage, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag, nullptr);
+        else launchRhsPlainK<T, false, false>(s, stage, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag, nullptr);
     }
-    launchRhsPlainK<T, false>(s, stage, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho, fFlag, nullptr);
 }
 
 namespace {
@@ -346,14 +366,14 @@ template void launchRhoCommit<float>(hipStream_t, const Layout&, float*, const f
                                      const int32_t*);
 
 template void launchDerive<double>(hipStream_t, const Layout&, const Params&, const double*, double*, const uint8_t*,
-                                   const Window&, const double*, const double*);
+                                   const Window&, const double*, const double*, int);
 template void launchDerive<float>(hipStream_t, const Layout&, const Params&, const float*, float*, const uint8_t*,
-                                  const Window&, const float*, const float*);
+                                  const Window&, const float*, const float*, int);
 template void launchRhsPlain<double>(hipStream_t, int, const Layout&, const Params&, const double*, const double*,
                                      const double*, double*, double*, const uint8_t*, const double*, const double*,
-                                     const int32_t*, int8_t*);
+                                     const int32_t*, int8_t*, bool);
 template void launchRhsPlain<float>(hipStream_t, int, const Layout&, const Params&, const float*, const float*,
                                     const float*, float*, float*, const uint8_t*, const double*, const float*,
-                                    const int32_t*, int8_t*);
+                                    const int32_t*, int8_t*, bool);
 
 }  // namespace nsfsi

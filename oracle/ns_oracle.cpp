@@ -129,9 +129,22 @@ public:
         double T = (P.gamma * p - c.rho) / (1 + c.rho);
         return {u, v, w, p, T};
     }
+    // x^1.5 (src/Solver.cpp:476, src/FlowVariableGroupStructs.h:183): the reference's std::pow by default;
+    // with crPow the correctly rounded value (Dekker's double-length x*sqrt(x), as glibc's pow returned it
+    // before version 2.28; the current glibc pow is within 0.52 ulp and differs from the correctly rounded
+    // value by one ulp for about one input in a thousand near 1).
+    static double pow15cr(double x) {
+        const double s = std::sqrt(x);
+        const double r = std::fma(-s, s, x);
+        const double e = r / (2.0 * s);
+        const double p = x * s;
+        const double pe = std::fma(x, s, -p);
+        return p + (pe + x * e);
+    }
+    double pow15(double x) const { return crPow ? pow15cr(x) : std::pow(x, 1.5); }
     Trans deriveTrans(const Prim& q) const {
         double ScPlusOne = 1 + P.sutherlands_C2 / P.T_0;
-        double mu = std::pow(1 + q.T, 1.5) * ScPlusOne / (P.Re * (q.T + ScPlusOne));
+        double mu = pow15(1 + q.T) * ScPlusOne / (P.Re * (q.T + ScPlusOne));
         double kappa = mu / ((P.gamma - 1) * P.Pr);
         return {mu, kappa};
     }
@@ -505,7 +518,7 @@ public:
             double u = cons[1][n] / (rho + 1), v = cons[2][n] / (rho + 1), w = cons[3][n] / (rho + 1);
             double p = gm1 * (cons[4][n] - (rho + 1) / 2 * (u * u + v * v + w * w));
             double T = (P.gamma * p - rho) / (1 + rho);
-            double mu = std::pow(1 + T, 1.5) * ScP1 / (P.Re * (T + ScP1));
+            double mu = pow15(1 + T) * ScP1 / (P.Re * (T + ScP1));
             prim[0][n] = u; prim[1][n] = v; prim[2][n] = w; prim[3][n] = p; prim[4][n] = T;
             trans[0][n] = mu; trans[1][n] = mu * pf;
         }
@@ -834,6 +847,7 @@ public:
     double t = 0, dt = 0;
     uint64_t timeLevel = 0;
     bool nonFinite = false;
+    bool crPow = false;
     std::string err;
 };
 
@@ -1067,6 +1081,17 @@ double ns_oracle_update_dt(void* h, double tnow) {
     o->updateTimeStepSize(tnow);
     return o->dt;
 }
+
+/* Selects x^1.5 in the viscosity law: 0 = std::pow (the reference's call), 1 = correctly rounded. */
+int ns_oracle_set_cr_pow(void* h, int on) {
+    auto* o = (Oracle*)h;
+    if (!o) return NS_ERR_ARGUMENT;
+    o->crPow = on != 0;
+    return NS_OK;
+}
+
+/* Unit hook: x^1.5 as the oracle evaluates it (mode 0 std::pow, 1 correctly rounded). */
+double ns_oracle_pow15(double x, int mode) { return mode ? Oracle::pow15cr(x) : std::pow(x, 1.5); }
 
 /* Unit hook: 8x8 full-pivot LU solve of A x = b (A row-major). */
 void ns_oracle_lu8_solve(const double* A, const double* b, double* x) {

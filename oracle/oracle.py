@@ -62,6 +62,9 @@ def lib():
     L.ns_oracle_lu8_solve.restype = None
     L.ns_oracle_uniform01.argtypes = [C.c_uint64, C.c_uint64]
     L.ns_oracle_uniform01.restype = C.c_double
+    L.ns_oracle_set_cr_pow.argtypes = [H, C.c_int]
+    L.ns_oracle_pow15.argtypes = [C.c_double, C.c_int]
+    L.ns_oracle_pow15.restype = C.c_double
     L.ns_oracle_set_time.argtypes = [H, C.c_double, C.c_uint64]
     L.ns_oracle_filter_now.argtypes = [H]
     L.ns_oracle_rho_filter.argtypes = [H]
@@ -185,6 +188,10 @@ class Oracle:
     def old_field(self, q) -> np.ndarray:
         return self.field(12 + int(q))
 
+    def set_correctly_rounded_pow(self, on=True):
+        """x^1.5 of the viscosity law correctly rounded (True) or by std::pow, the reference's call (False)."""
+        self._check(lib().ns_oracle_set_cr_pow(self._h, 1 if on else 0))
+
     def set_time(self, t, level):
         self._check(lib().ns_oracle_set_time(self._h, float(t), int(level)))
 
@@ -219,6 +226,10 @@ def lu8_solve(A, b):
     x = np.zeros(8)
     lib().ns_oracle_lu8_solve(_d(A), _d(b), _d(x))
     return x
+
+
+def pow15(x, correctly_rounded):
+    return lib().ns_oracle_pow15(float(x), 1 if correctly_rounded else 0)
 
 
 def uniform01(seed, counter):

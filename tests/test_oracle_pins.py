@@ -482,3 +482,23 @@ def test_perturbation_generator_statistics():
     assert np.all((u >= 0) & (u < 1))
     assert abs(u.mean() - 0.5) < 0.01
     assert abs(u.var() - 1 / 12) < 0.005
+
+
+def test_pow15_correctly_rounded():
+    """The oracle's correctly rounded x^1.5 (its crPow mode, the one the device's FP64 path uses) equals x*sqrt(x)
+    evaluated to 50 digits and rounded once, on 20000 inputs around 1 + T' and some far from 1; the reference's
+    std::pow (glibc's pow is within 0.52 ulp) is at most one ulp away."""
+    from decimal import Decimal, getcontext
+
+    getcontext().prec = 50
+    rng = np.random.default_rng(15)
+    xs = np.concatenate([1 + rng.uniform(-0.02, 0.02, 20000), rng.uniform(0.5, 4.0, 2000), [1.0, 4.0, 0.25, 2.0]])
+    diff_glibc = 0
+    for x in xs:
+        d = Decimal(float(x))
+        exact = float(d * d.sqrt())  # one rounding of the 50-digit value
+        assert oracle.pow15(x, True) == exact, x
+        g = oracle.pow15(x, False)
+        assert abs(g - exact) <= np.spacing(exact)
+        diff_glibc += g != exact
+    print(f"std::pow(x, 1.5) differs from the correctly rounded value at {diff_glibc} of {xs.size} inputs")
