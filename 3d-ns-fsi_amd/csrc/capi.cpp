@@ -253,6 +253,20 @@ ns_status ns_set_option(ns_handle* h, const char* name, int64_t value) {
     });
 }
 
+ns_status ns_save(ns_handle* h, const char* path) {
+    return guarded(h, [&] {
+        if (!path) throw std::invalid_argument("null path");
+        live(h).save(path);
+    });
+}
+
+ns_status ns_restore(ns_handle* h, const char* path) {
+    return guarded(h, [&] {
+        if (!path) throw std::invalid_argument("null path");
+        live(h).restore(path);
+    });
+}
+
 ns_status ns_get_tile(ns_handle* h, int32_t* ty, int32_t* kc) {
     return guarded(h, [&] {
         auto& c = live(h);

@@ -141,6 +141,12 @@ public:
     void adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* gradU0, double* gradInflow,
                      double* gradRe, double* dts);
     double referenceArea() const;
+    // Checkpoint of everything a later step reads: both state buffers (the "Old" values of non-interior nodes
+    // live in A), the stored primitive fields (FP64), the pre-filter tables, the FP32 compensation carry,
+    // the untouched solid nodes' initial primitives, the step scalars and the norm and force histories.
+    // restore() needs a case created from the same grid, boundaries, body and physics and initialised.
+    void save(const std::string& path);
+    void restore(const std::string& path);
     double t = 0, dt = 0;
     uint64_t level = 0;
     bool nonFinite = false;

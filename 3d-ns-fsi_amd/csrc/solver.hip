@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -909,6 +910,136 @@ void Case::getField(int which, double* out) {
     }
     hipCheck(hipStreamSynchronize(stream), "sync");
     copyOut(prim, which - 5);
+}
+
+// ------------------------------------------------------------------------------------------------ checkpoints
+
+namespace {
+
+constexpr uint64_t kCkptMagic = 0x31544b5049534e4eULL;  // "NNSIPKT1"
+constexpr uint32_t kCkptVersion = 1;
+
+// Configuration fields a checkpoint must agree on: the grid, boundaries, body, initial-state generator
+// and physics (stopping, output and stability-limit settings may change between runs).
+struct CkptKey {
+    int32_t NI, NJ, NK, precision, kahan, bc[6], body_type, body_axis;
+    double body_centre[3], body_radius, gamma, Pr, Re, sutherlands_C2, M_0, T_0, L_x, L_y, L_z, inlet_ramp_time;
+};
+CkptKey ckptKey(const ns_config& c) {
+    CkptKey k;
+    std::memset(&k, 0, sizeof(k));
+    k.NI = c.NI; k.NJ = c.NJ; k.NK = c.NK; k.precision = c.precision; k.kahan = c.precision == NS_FP32 ? c.kahan : 0;
+    for (int q = 0; q < 6; ++q) k.bc[q] = c.bc[q];
+    k.body_type = c.body_type;
+    k.body_axis = c.body_type == NS_BODY_CYLINDER ? c.body_axis : 0;
+    for (int q = 0; q < 3; ++q) k.body_centre[q] = c.body_centre[q];
+    k.body_radius = c.body_radius; k.gamma = c.gamma; k.Pr = c.Pr; k.Re = c.Re; k.sutherlands_C2 = c.sutherlands_C2;
+    k.M_0 = c.M_0; k.T_0 = c.T_0; k.L_x = c.L_x; k.L_y = c.L_y; k.L_z = c.L_z; k.inlet_ramp_time = c.inlet_ramp_time;
+    return k;
+}
+
+struct CkptHeader {
+    uint64_t magic;
+    uint32_t version, primsStored;
+    CkptKey key;
+    uint64_t fstride, windowSize, nInactive, nNorms, nForces;
+    int32_t conv_history, pad;
+};
+
+}  // namespace
+
+void Case::save(const std::string& path) {
+    if (!initialized) throw StateError("save before init");
+    readScalars();  // drains the history rings into normsHost / forceHost
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    CkptHeader hd;
+    std::memset(&hd, 0, sizeof(hd));
+    hd.magic = kCkptMagic;
+    hd.version = kCkptVersion;
+    hd.primsStored = primsStored ? 1 : 0;
+    hd.key = ckptKey(cfg);
+    hd.fstride = L.fstride;
+    hd.windowSize = W.size();
+    hd.nInactive = (uint64_t)nInactive;
+    hd.nNorms = normsHost[0].size();
+    hd.nForces = forceHost.size() / 4;
+    hd.conv_history = cfg.conv_history;
+    std::ofstream f(path, std::ios::binary | std::ios::trunc);
+    if (!f) throw std::invalid_argument("cannot open checkpoint file for writing: " + path);
+    auto put = [&](const void* p, size_t n) {
+        f.write((const char*)p, (std::streamsize)n);
+        if (!f) throw std::runtime_error("checkpoint write failed: " + path);
+    };
+    std::vector<char> host;
+    auto putDevice = [&](const void* d, size_t n) {
+        host.resize(std::max<size_t>(n, 1));
+        if (n) hipCheck(hipMemcpy(host.data(), d, n, hipMemcpyDeviceToHost), "checkpoint copy");
+        put(host.data(), n);
+    };
+    put(&hd, sizeof(hd));
+    const size_t fb = 5 * L.fstride * eb, wb = std::max<size_t>(1, W.size()) * eb;
+    putDevice(U, fb);
+    putDevice(A, fb);
+    if (!fp32) putDevice(prim, 7 * L.fstride * eb);
+    putDevice(rhoPre, wb);
+    putDevice(rhoEPre, wb);
+    if (comp) putDevice(comp, 5 * L.fstride * sizeof(int8_t));
+    putDevice(inactiveIC, 5 * std::max<int64_t>(1, nInactive) * sizeof(double));
+    StepScalars st;
+    hipCheck(hipMemcpy(&st, scal, sizeof(st), hipMemcpyDeviceToHost), "checkpoint scalars");
+    put(&st, sizeof(st));
+    for (int v = 0; v < 5; ++v) put(normsHost[v].data(), normsHost[v].size() * sizeof(double));
+    put(forceHost.data(), forceHost.size() * sizeof(double));
+}
+
+void Case::restore(const std::string& path) {
+    if (!initialized) throw StateError("restore before init");
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::invalid_argument("cannot open checkpoint file: " + path);
+    auto get = [&](void* p, size_t n) {
+        f.read((char*)p, (std::streamsize)n);
+        if (!f) throw std::invalid_argument("checkpoint file is truncated: " + path);
+    };
+    CkptHeader hd;
+    get(&hd, sizeof(hd));
+    if (hd.magic != kCkptMagic || hd.version != kCkptVersion) throw std::invalid_argument("not a checkpoint file: " + path);
+    const CkptKey want = ckptKey(cfg);
+    if (std::memcmp(&hd.key, &want, sizeof(want)) != 0)
+        throw std::invalid_argument("checkpoint was written by a case with another grid, boundaries, body or physics");
+    if (hd.fstride != L.fstride || hd.windowSize != W.size() || hd.nInactive != (uint64_t)nInactive ||
+        hd.conv_history != cfg.conv_history)
+        throw std::invalid_argument("checkpoint layout does not match this case");
+    dropGraphs();
+    std::vector<char> host;
+    auto getDevice = [&](void* d, size_t n) {
+        host.resize(std::max<size_t>(n, 1));
+        get(host.data(), n);
+        if (n) hipCheck(hipMemcpy(d, host.data(), n, hipMemcpyHostToDevice), "restore copy");
+    };
+    const size_t fb = 5 * L.fstride * eb, wb = std::max<size_t>(1, W.size()) * eb;
+    getDevice(U, fb);
+    getDevice(A, fb);
+    if (!fp32) getDevice(prim, 7 * L.fstride * eb);
+    getDevice(rhoPre, wb);
+    getDevice(rhoEPre, wb);
+    if (comp) getDevice(comp, 5 * L.fstride * sizeof(int8_t));
+    getDevice(inactiveIC, 5 * std::max<int64_t>(1, nInactive) * sizeof(double));
+    StepScalars st;
+    get(&st, sizeof(st));
+    // the device rings restart empty; the histories so far are kept on the host
+    std::vector<double> norms[5], forces(hd.nForces * 4);
+    for (int v = 0; v < 5; ++v) {
+        norms[v].resize(hd.nNorms);
+        get(norms[v].data(), hd.nNorms * sizeof(double));
+    }
+    get(forces.data(), forces.size() * sizeof(double));
+    char extra;
+    if (f.read(&extra, 1)) throw std::invalid_argument("checkpoint file has trailing data: " + path);
+    hipCheck(hipMemcpy(scal, &st, sizeof(st), hipMemcpyHostToDevice), "restore scalars");
+    for (int v = 0; v < 5; ++v) normsHost[v] = norms[v];
+    forceHost = forces;
+    primsStored = hd.primsStored != 0;
+    readScalars();
 }
 
 void Case::nodeTypes(uint8_t* out) {

@@ -114,6 +114,8 @@ def lib():
     L.ns_profile.argtypes = [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.ns_set_option.argtypes = [H, C.c_char_p, C.c_int64]
     L.ns_get_tile.argtypes = [H, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.ns_save.argtypes = [H, C.c_char_p]
+    L.ns_restore.argtypes = [H, C.c_char_p]
     L.ns_last_error.argtypes = [H]
     L.ns_last_error.restype = C.c_char_p
     L.ns_destroy.argtypes = [H]
@@ -144,7 +146,8 @@ def lib():
     for name in ("ns_config_parse", "ns_create", "ns_init", "ns_set_initial_field", "ns_step", "ns_run", "ns_get_time",
                  "ns_get_field", "ns_get_node_types", "ns_get_counts", "ns_get_ghosts", "ns_get_filter_nodes",
                  "ns_norm_history", "ns_force_history", "ns_set_dt_schedule", "ns_set_inflow_profile",
-                 "ns_adjoint_drag", "ns_totals", "ns_get_geometry", "ns_profile", "ns_set_option", "ns_get_tile"):
+                 "ns_adjoint_drag", "ns_totals", "ns_get_geometry", "ns_profile", "ns_set_option", "ns_get_tile",
+                 "ns_save", "ns_restore"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -468,6 +471,15 @@ class Solver:
         c = self.config
         return {"J": J.value, "grad_u0": gu.reshape(5, c.NK, c.NJ, c.NI), "grad_inflow": gq.reshape(shp),
                 "grad_re": gre.value, "dts": dts}
+
+    def save(self, path):
+        """Checkpoint of the run (ns_save): a restored run continues bit for bit."""
+        self._check(lib().ns_save(self._h, os.fsencode(path)))
+
+    def restore(self, path):
+        """Loads a checkpoint written by a case of the same grid, boundaries, body and physics (ns_restore);
+        call after initialize()."""
+        self._check(lib().ns_restore(self._h, os.fsencode(path)))
 
     def referenceArea(self) -> float:
         """Frontal area for force coefficients: pi r^2 (sphere), 2 r times the span of FluidInterior planes

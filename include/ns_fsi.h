@@ -233,6 +233,18 @@ ns_status ns_profile(ns_handle* h, int32_t nsteps, double ms[4], int32_t launche
  * Results do not depend on the tile shape (the per-node arithmetic is the same). */
 ns_status ns_set_option(ns_handle* h, const char* name, int64_t value);
 
+/* Checkpoint and restart (no reference counterpart: the reference writes output snapshots only). ns_save writes
+ * to the file `path` everything a later step reads: the state buffers (including the previous step's values the
+ * reference keeps in its "Old" arrays), the stored primitive fields, the pre-filter density and energy of the
+ * immersed-filter nodes, the FP32 compensation carry, the initial primitives of untouched solid nodes, t, dt,
+ * timeLevel and the norm and force histories. ns_restore reads such a file into an initialised handle created
+ * with the same grid, boundary types, body, initial-state generator and physical parameters (stopping, output
+ * and stability settings may differ; NS_ERR_ARGUMENT otherwise): a restarted run continues bit for bit as the
+ * uninterrupted one would. Run-time options, tile, prescribed step sizes and the inlet profile are not part of
+ * a checkpoint. */
+ns_status ns_save(ns_handle* h, const char* path);
+ns_status ns_restore(ns_handle* h, const char* path);
+
 /* Tile shape in use by the fused stage (after autotuning, the chosen one). */
 ns_status ns_get_tile(ns_handle* h, int32_t* ty, int32_t* kc);
 

@@ -287,6 +287,16 @@ public:
         return history;
     }
 
+    // Checkpoint and restart (ns_save / ns_restore; no reference counterpart).
+    void saveCheckpoint(const std::string& path) { nsfsi_host::check(h, ns_save(h, path.c_str()), "ns_save"); }
+    void restoreCheckpoint(const std::string& path, double& t, ulong& timeLevel) {
+        nsfsi_host::check(h, ns_restore(h, path.c_str()), "ns_restore");
+        uint64_t lv = 0;
+        nsfsi_host::check(h, ns_get_time(h, &t, &lv, &dt), "ns_get_time");
+        timeLevel = (ulong)lv;
+        ++version;
+    }
+
     ns_handle* handle() const { return h; }
 
     const ConfigSettings params;
