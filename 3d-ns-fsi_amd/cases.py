@@ -71,6 +71,30 @@ def sweep64(n=64, steps=500, seed=4, precision="fp32"):
     return out
 
 
+def two_bodies(steps=10, precision="fp64"):
+    """A sphere (r 0.45) upstream of a z-axis cylinder (r 0.4), 48x30x26 nodes, slip side walls, Mach 0.3,
+    Re 300, Gaussian perturbation sigma 1e-3: two immersed bodies, classified and applied in list order."""
+    return nsfsi.make_config(
+        NI=48, NJ=30, NK=26, L_x=4.8, L_y=3.0, L_z=2.6, bc=SIDE_WALLS, body_type="sphere",
+        body_centre=(1.2, 1.5, 1.3), body_radius=0.45,
+        extra_bodies=[dict(type="cylinder", axis="z", centre=(3.3, 1.4, 0.0), radius=0.4)],
+        stop_criterion="timesteps", stop_time_level=steps, filter_intervals=[0], filter_change_times=[],
+        ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=23, M_0=0.3, Re=300.0, conv_stability_limit=0.5,
+        precision=precision)
+
+
+def three_bodies(steps=10, precision="fp64"):
+    """Two spheres side by side in y (r 0.35) and a z-axis cylinder downstream (r 0.4), 48x40x26 nodes."""
+    return nsfsi.make_config(
+        NI=48, NJ=40, NK=26, L_x=4.8, L_y=4.0, L_z=2.6, bc=SIDE_WALLS, body_type="sphere",
+        body_centre=(1.0, 1.0, 1.3), body_radius=0.35,
+        extra_bodies=[dict(type="sphere", centre=(1.0, 3.0, 1.3), radius=0.35),
+                      dict(type="cylinder", axis="z", centre=(3.4, 2.0, 0.0), radius=0.4)],
+        stop_criterion="timesteps", stop_time_level=steps, filter_intervals=[0], filter_change_times=[],
+        ic_type="freestream_gaussian", ic_amplitude=1e-3, ic_seed=29, M_0=0.3, Re=300.0, conv_stability_limit=0.5,
+        precision=precision)
+
+
 WORKLOADS = {
     "sphere48": sphere48,
     "cylinder256": cylinder256,

@@ -34,7 +34,8 @@ struct Value {
     double f = 0;
     bool b = false;
     std::string s;
-    std::vector<Value> items;
+    std::vector<Value> items;         // array elements, or group members (names below)
+    std::vector<std::string> names;   // group member names, in file order
 };
 
 struct ParseError : std::runtime_error { using std::runtime_error::runtime_error; };
@@ -135,7 +136,11 @@ private:
             while (*p && *p != '}') { parseSetting(inner); skip(); }
             if (*p != '}') fail("unterminated group");
             ++p;
-            return v;  // groups are accepted and ignored: the reference reads top-level keys only
+            for (auto& kv : inner) {  // the reference reads top-level keys only; groups serve the extension keys
+                v.names.push_back(kv.first);
+                v.items.push_back(kv.second);
+            }
+            return v;
         }
         // booleans
         auto word = [&](const char* w) {
@@ -219,6 +224,45 @@ private:
     const std::map<std::string, Value>& m;
 };
 
+// One entry of the extraBodies list: { type = "sphere" | "cylinder"; axis = "x" | "y" | "z"; centre = [x, y, z];
+// radius = r; } (axis only for a cylinder).
+ns_body bodyFromGroup(const Value& g, int q) {
+    const std::string where = "extraBodies entry " + std::to_string(q) + ": ";
+    if (g.kind != Value::Group) throw ParseError(where + "must be a group { ... }");
+    ns_body b{};
+    b.type = -1;
+    bool haveCentre = false, haveRadius = false;
+    for (size_t m = 0; m < g.names.size(); ++m) {
+        const std::string& n = g.names[m];
+        const Value& v = g.items[m];
+        if (n == "type") {
+            if (v.kind != Value::String) throw ParseError(where + "type must be a string");
+            if (v.s == "cylinder") b.type = NS_BODY_CYLINDER;
+            else if (v.s == "sphere") b.type = NS_BODY_SPHERE;
+            else throw ParseError(where + "type must be cylinder or sphere");
+        } else if (n == "axis") {
+            if (v.kind != Value::String || v.s.size() != 1 || v.s.find_first_of("xyz") != 0)
+                throw ParseError(where + "axis must be x, y or z");
+            b.axis = v.s[0] - 'x';
+        } else if (n == "centre") {
+            if (v.kind != Value::Array || v.items.size() != 3) throw ParseError(where + "centre needs 3 floats");
+            for (int a = 0; a < 3; ++a) {
+                if (v.items[a].kind != Value::Float) throw ParseError(where + "centre must hold floats");
+                b.centre[a] = v.items[a].f;
+            }
+            haveCentre = true;
+        } else if (n == "radius") {
+            if (v.kind != Value::Float) throw ParseError(where + "radius must be a floating-point value");
+            b.radius = v.f;
+            haveRadius = true;
+        } else {
+            throw ParseError(where + "unknown member " + n);
+        }
+    }
+    if (b.type < 0 || !haveCentre || !haveRadius) throw ParseError(where + "needs type, centre and radius");
+    return b;
+}
+
 int bcFromString(const std::string& s) {
     if (s == "inlet") return NS_BC_INLET;
     if (s == "outlet") return NS_BC_OUTLET;
@@ -251,6 +295,14 @@ std::string validateConfig(const ns_config& c) {
     if (c.body_type < 0 || c.body_type > 2) return "unknown body type";
     if (c.body_type == NS_BODY_CYLINDER && (c.body_axis < 0 || c.body_axis > 2)) return "bodyAxis must be x, y or z";
     if (c.body_type != NS_BODY_NONE && !(c.body_radius > 0)) return "bodyRadius must be positive";
+    if (c.n_extra_bodies < 0 || c.n_extra_bodies > NS_MAX_BODIES - 1) return "at most 7 extra bodies";
+    if (c.n_extra_bodies > 0 && c.body_type == NS_BODY_NONE) return "extra bodies need a first body (bodyType)";
+    for (int q = 0; q < c.n_extra_bodies; ++q) {
+        const ns_body& b = c.extra_bodies[q];
+        if (b.type != NS_BODY_CYLINDER && b.type != NS_BODY_SPHERE) return "extra body type must be cylinder or sphere";
+        if (b.type == NS_BODY_CYLINDER && (b.axis < 0 || b.axis > 2)) return "extra cylinder axis must be x, y or z";
+        if (!(b.radius > 0)) return "extra body radius must be positive";
+    }
     if (c.ic_type < 0 || c.ic_type > 3) return "unknown initial condition";
     if (c.precision != NS_FP64 && c.precision != NS_FP32) return "precision must be fp64 or fp32";
     if (c.stop_criterion != 0 && c.stop_criterion != 1) return "stopCriterion must be timesteps or end_time";
@@ -404,6 +456,12 @@ extern "C" ns_status ns_config_parse(const char* text, ns_config* out, char* err
             }
         }
         if (r.has("bodyRadius")) c.body_radius = r.real("bodyRadius");
+        if (r.has("extraBodies")) {
+            const auto& eb = r.array("extraBodies");
+            if (eb.size() > NS_MAX_BODIES - 1) throw ParseError("extraBodies holds at most 7 bodies");
+            c.n_extra_bodies = (int32_t)eb.size();
+            for (size_t q = 0; q < eb.size(); ++q) c.extra_bodies[q] = bodyFromGroup(eb[q], (int)q);
+        }
         if (r.has("initialCondition")) {
             std::string ic = r.str("initialCondition");
             if (ic == "stagnation") c.ic_type = NS_IC_STAGNATION;

@@ -23,6 +23,8 @@
 namespace nsfsi {
 
 double Case::referenceArea() const {
+    const BodyDesc& B = bodies.b[0];
+    if (bodies.n != 1) throw Unsupported("the drag objective covers one immersed body");
     if (B.type == NS_BODY_SPHERE) return M_PI * B.r * B.r;
     if (B.type == NS_BODY_CYLINDER) {
         const int n = B.axis == 0 ? L.NI : (B.axis == 1 ? L.NJ : L.NK);
@@ -39,6 +41,8 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
     if (level != 0) throw StateError("the adjoint starts from the initial state (time level 0)");
     if (!filterDisabled()) throw Unsupported("the adjoint does not cover the periodic density filter");
     if (nGhost == 0) throw Unsupported("the drag objective needs an immersed body");
+    if (bodies.n != 1) throw Unsupported("the drag objective covers one immersed body");
+    if (nFixed > 0) throw Unsupported("the adjoint does not cover bodies whose solid nodes reach a symmetry face");
     if (nsteps < 1 || n0 < 0 || n1 < n0 || n1 > nsteps || ckpt < 1)
         throw std::invalid_argument("need 1 <= nsteps, 0 <= n0 <= n1 <= nsteps, ckpt >= 1");
     checkSchedule(nsteps);  // a prescribed schedule must cover the forward run

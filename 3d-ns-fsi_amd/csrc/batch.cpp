@@ -27,6 +27,7 @@ Batch::Batch(const ns_config* cfgs, int n) {
         const ns_config& k = cfgs[c];
         if (k.precision != NS_FP32) throw Unsupported("batched cases run in FP32 (precision = fp32)");
         if (k.kahan != cfgs[0].kahan) throw Unsupported("all batched cases need the same kahan setting");
+        if (k.n_extra_bodies != 0) throw Unsupported("batched cases hold at most one immersed body");
         cases.push_back(std::make_unique<Case>(k));
         if (!cases.back()->filterDisabled())
             throw Unsupported("the periodic density filter is not available in batched runs (filterIntervals = [0])");
@@ -64,7 +65,11 @@ void Batch::dropGraphs() {
 }
 
 void Batch::init() {
-    for (auto& c : cases) c->init();
+    for (auto& c : cases) {
+        c->init();
+        if (c->nFixed > 0)
+            throw Unsupported("a body whose solid nodes reach a symmetry face is not available in batched runs");
+    }
     const int n = (int)cases.size();
     // device views (buffers in their initial roles; parity 0)
     std::vector<CaseView<float>> views(n);

@@ -79,7 +79,7 @@ public:
     Layout L{};
     Params P{};
     Window W{};
-    BodyDesc B{};
+    BodySet bodies{};
     EdgeFace faces[6]{};  // application order (reverse of the claim order)
     // faces[fusedOutlet] (or -1): the outlet face that the fused stage kernel applies itself; bcFaces are the
     // faces with that one left out, for the edge-face pass after a fused stage
@@ -91,10 +91,23 @@ public:
     // classification results
     uint8_t* cls = nullptr;
     int64_t nGhost = 0, nFilter = 0, nInactive = 0;
+    // body b's entries of the ghost and filter lists: [ghostOff[b], ghostOff[b+1]), [filterOff[b], filterOff[b+1])
+    int64_t ghostOff[NS_MAX_BODIES + 1]{}, filterOff[NS_MAX_BODIES + 1]{};
     int64_t* ghostList = nullptr;
     int64_t* filterList = nullptr;
     int64_t* inactiveList = nullptr;
     double* inactiveIC = nullptr;
+    // Boundary nodes whose conserved values are constant: symmetry mirrors (possibly through periodic copies
+    // and further mirrors) of untouched SolidInactive nodes, whose stored primitives the reference keeps at
+    // their initial values. fixedNodes: layout offsets; fixedSrc: layout offset of the untouched source node;
+    // fixedMirror: bit a set = velocity component a negated; fixedVals: 5 conserved values per node.
+    int64_t nFixed = 0;
+    std::vector<int64_t> fixedHostNodes, fixedSrc;
+    std::vector<uint8_t> fixedMirror;
+    int64_t* fixedNodes = nullptr;
+    double* fixedVals = nullptr;
+    void findFixedBoundary();
+    template <class T> void computeFixedBoundary(const std::vector<T>& h);
     GhostTables G{};
 
     // fields (element type float or double)

@@ -25,6 +25,13 @@ struct BodyDesc {
     Window box;         // body bounding box (getSphereBoundingBox / getCylinderBoundingBox)
 };
 
+// The immersed bodies in the order of the reference's body list (src/Mesh.cpp:52-57); their boxes are
+// pairwise disjoint.
+struct BodySet {
+    int n;
+    BodyDesc b[NS_MAX_BODIES];
+};
+
 // Per-ghost data of the image-point interpolation (frozen geometry).
 struct GhostTables {
     int64_t* node;      // layout offset of the ghost node
@@ -40,18 +47,25 @@ struct GhostTables {
 };
 
 // ---- classification and ghost geometry (mesh.hip, compiled without FMA contraction)
-void launchClassify(hipStream_t s, const Layout& L, uint8_t* cls, const int faceNodeType[6], const BodyDesc& B,
+void launchClassify(hipStream_t s, const Layout& L, uint8_t* cls, const int faceNodeType[6], const BodySet& bodies,
                     const Params& P);
 void launchMarkGhosts(hipStream_t s, const Layout& L, uint8_t* cls, const Window& W);
 // Compacts node ids n = (k*NJ+j)*NI+i of window nodes whose class satisfies (cls & mask) == value, in
 // ascending order. blockCounts must hold ceil(W.size()/256) ints. Returns the count (synchronises).
 int64_t compactWindow(hipStream_t s, const Layout& L, const uint8_t* cls, const Window& W, uint8_t mask, uint8_t value,
                       int64_t* out, int32_t* blockCounts, int32_t* blockOffsets);
+// Image points of the ghosts of body B; a SolidInactive box corner becomes a ghost. A corner outside B's own
+// box sets error bit 2 (the box reaches another body's solid nodes).
 void launchImagePoints(hipStream_t s, const Layout& L, uint8_t* cls, const Params& P, const BodyDesc& B,
                        const int64_t* ghosts, int64_t n, double* bi, double* ip, int32_t* changed, int32_t* error);
 void launchFillGhostMap(hipStream_t s, const Layout& L, const Window& W, const int64_t* ghosts, int64_t n, int32_t* gmap);
+// Ghost geometry of one body's ghosts, entries [base, base + n) of the ghost list; G points at entry base.
+// A ghost corner of another body sets error bit 2.
 void launchGhostGeometry(hipStream_t s, const Layout& L, const uint8_t* cls, const Params& P, const Window& W,
-                         const int64_t* ghosts, int64_t n, const int32_t* gmap, GhostTables G, int32_t* error);
+                         const int64_t* ghosts, int64_t base, int64_t n, const int32_t* gmap, GhostTables G,
+                         int32_t* error);
+// The tables of entries [base, ...) of a ghost list.
+GhostTables ghostTablesFrom(const GhostTables& G, int64_t base);
 
 // ---- boundary conditions on a state buffer (mesh.hip)
 // Whether two edge faces can be applied in one launch: disjoint owned boxes and neither reads a node the

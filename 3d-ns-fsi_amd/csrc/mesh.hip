@@ -64,7 +64,8 @@ __device__ __forceinline__ void normalProbe(const BodyDesc& B, double gx, double
     pz = cz * lf;
 }
 
-__global__ void k_classify(Layout L, uint8_t* cls, int t0, int t1, int t2, int t3, int t4, int t5, BodyDesc B, Params P) {
+__global__ void k_classify(Layout L, uint8_t* cls, int t0, int t1, int t2, int t3, int t4, int t5, BodySet bodies,
+                           Params P) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y;
     int k = blockIdx.z;
@@ -78,7 +79,10 @@ __global__ void k_classify(Layout L, uint8_t* cls, int t0, int t1, int t2, int t
     else if (i == 0) t = t0;
     else if (i == L.NI - 1) t = t1;
     uint8_t flags = 0;
-    if (B.type != NS_BODY_NONE && B.box.contains(i, j, k)) {
+    // the body whose (disjoint) box holds the node; the others never look at it
+    for (int b = 0; b < bodies.n; ++b) {
+        const BodyDesc& B = bodies.b[b];
+        if (!B.box.contains(i, j, k)) continue;
         double x, y, z;
         nodePos(P, i, j, k, x, y, z);
         double dx = x - B.cx, dy = y - B.cy, dz = z - B.cz, d;
@@ -89,6 +93,7 @@ __global__ void k_classify(Layout L, uint8_t* cls, int t0, int t1, int t2, int t
         // getSolidAndFilterNodesIn{Cylinder,Sphere}, src/Boundary.cpp:1005-1012, :1094-1101
         if (d < B.r - B.mpb) t = NS_SOLID_INACTIVE;
         else if (d < B.r + B.filterWidth && t == NS_FLUID_INTERIOR) flags |= CLS_FILTER;
+        break;
     }
     cls[L.off(i, j, k)] = (uint8_t)(t | flags);
 }
@@ -182,6 +187,7 @@ __global__ void k_image_points(Layout L, uint8_t* cls, Params P, BodyDesc B, con
         size_t o = L.off(ii, jj, kk);
         uint8_t v = cls[o];
         if ((v & CLS_TYPE_MASK) == NS_SOLID_INACTIVE) {
+            if (!B.box.contains(ii, jj, kk)) { atomicOr(error, 2); continue; }
             cls[o] = (uint8_t)((v & ~CLS_TYPE_MASK) | NS_SOLID_GHOST);
             atomicAdd(changed, 1);
         }
@@ -228,8 +234,8 @@ __device__ void lu8Factor(double* A, int8_t* rp, int8_t* cp) {
 
 // Image-point geometry, corner tables and both Vandermonde factorisations for every ghost
 // (setInterpolationValues + populateVandermonde*, src/Boundary.cpp:655-728, :788-817).
-__global__ void k_ghost_geometry(Layout L, const uint8_t* cls, Params P, Window W, const int64_t* ghosts, int64_t n,
-                                 const int32_t* gmap, GhostTables G, int32_t* error) {
+__global__ void k_ghost_geometry(Layout L, const uint8_t* cls, Params P, Window W, const int64_t* ghosts, int64_t base,
+                                 int64_t n, const int32_t* gmap, GhostTables G, int32_t* error) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     int i, j, k;
@@ -251,8 +257,9 @@ __global__ void k_ghost_geometry(Layout L, const uint8_t* cls, Params P, Window 
             nodePos(P, ii, jj, kk, X[c], Y[c], Z[c]);
         } else if (t == NS_SOLID_GHOST) {
             mask |= (uint8_t)(1u << c);
-            int32_t og = W.contains(ii, jj, kk) ? gmap[W.idx(ii, jj, kk)] : -1;
-            if (og < 0) { atomicOr(error, 2); return; }
+            // ghostNodeMap.at (src/Boundary.cpp:681): only this body's ghosts
+            int64_t og = W.contains(ii, jj, kk) ? (int64_t)gmap[W.idx(ii, jj, kk)] - base : -1;
+            if (og < 0 || og >= n) { atomicOr(error, 2); return; }
             double bx = G.bi[3 * og], by = G.bi[3 * og + 1], bz = G.bi[3 * og + 2];
             double vx = G.ip[3 * og] - bx, vy = G.ip[3 * og + 1] - by, vz = G.ip[3 * og + 2] - bz;
             double l = len3(vx, vy, vz);
@@ -642,9 +649,9 @@ inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ------------------------------------------------------------------------------------------------ launchers
 
-void launchClassify(hipStream_t s, const Layout& L, uint8_t* cls, const int ft[6], const BodyDesc& B, const Params& P) {
+void launchClassify(hipStream_t s, const Layout& L, uint8_t* cls, const int ft[6], const BodySet& bodies, const Params& P) {
     dim3 blk(128), grd(blocks(L.NI, 128), L.NJ, L.NK);
-    hipLaunchKernelGGL(k_classify, grd, blk, 0, s, L, cls, ft[0], ft[1], ft[2], ft[3], ft[4], ft[5], B, P);
+    hipLaunchKernelGGL(k_classify, grd, blk, 0, s, L, cls, ft[0], ft[1], ft[2], ft[3], ft[4], ft[5], bodies, P);
 }
 
 void launchMarkGhosts(hipStream_t s, const Layout& L, uint8_t* cls, const Window& W) {
@@ -686,9 +693,26 @@ void launchFillGhostMap(hipStream_t s, const Layout& L, const Window& W, const i
 }
 
 void launchGhostGeometry(hipStream_t s, const Layout& L, const uint8_t* cls, const Params& P, const Window& W,
-                         const int64_t* ghosts, int64_t n, const int32_t* gmap, GhostTables G, int32_t* error) {
+                         const int64_t* ghosts, int64_t base, int64_t n, const int32_t* gmap, GhostTables G,
+                         int32_t* error) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_ghost_geometry, dim3(blocks(n, 64)), dim3(64), 0, s, L, cls, P, W, ghosts, n, gmap, G, error);
+    hipLaunchKernelGGL(k_ghost_geometry, dim3(blocks(n, 64)), dim3(64), 0, s, L, cls, P, W, ghosts, base, n, gmap, G,
+                       error);
+}
+
+GhostTables ghostTablesFrom(const GhostTables& G, int64_t b) {
+    GhostTables o = G;
+    o.node = G.node + b;
+    o.corner = G.corner + 8 * b;
+    o.mask = G.mask + b;
+    o.rel = G.rel + 3 * b;
+    o.ip = G.ip + 3 * b;
+    o.bi = G.bi + 3 * b;
+    o.luD = G.luD + 64 * b;
+    o.luN = G.luN + 64 * b;
+    o.perm = G.perm + 32 * b;
+    o.boxLo = G.boxLo + 3 * b;
+    return o;
 }
 
 template <class T>

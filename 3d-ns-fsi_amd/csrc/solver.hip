@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -84,6 +85,15 @@ __global__ void k_gather_ic(Layout L, Params P, const T* U, const int64_t* nodes
     derivePrims(e, U[o], U[o + L.fstride], U[o + 2 * L.fstride], U[o + 3 * L.fstride], U[o + 4 * L.fstride], u, v, w, p, Tt,
                 mu);
     ic[5 * q] = u; ic[5 * q + 1] = v; ic[5 * q + 2] = w; ic[5 * q + 3] = p; ic[5 * q + 4] = mu;
+}
+
+// Boundary nodes with constant conserved values (symmetry mirrors of untouched solid nodes).
+template <class T>
+__global__ void k_apply_fixed(Layout L, const int64_t* nodes, const double* vals, int64_t n, T* S) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const size_t o = (size_t)nodes[q];
+    for (int c = 0; c < 5; ++c) S[o + (size_t)c * L.fstride] = (T)vals[5 * q + c];
 }
 
 // Primitive output for SolidInactive nodes: their initial values (never updated by the reference).
@@ -214,15 +224,16 @@ void Case::setupBoundaries() {
         bcFaces[fusedOutlet].i0 = 1;
         bcFaces[fusedOutlet].i1 = 0;
     }
-    // body
-    B = BodyDesc{};
-    B.type = cfg.body_type;
-    B.axis = cfg.body_axis;
-    B.cx = cfg.body_centre[0]; B.cy = cfg.body_centre[1]; B.cz = cfg.body_centre[2];
-    B.r = cfg.body_radius;
-    B.mpb = cfg.machine_precision_buffer;
+    // bodies, in the order of the reference's body list (src/Mesh.cpp:52-57)
+    bodies = BodySet{};
     W = Window{0, 0, 0, 0, 0, 0};
-    if (B.type != NS_BODY_NONE) {
+    auto addBody = [&](int type, int axis, const double* centre, double r) {
+        BodyDesc B{};
+        B.type = type;
+        B.axis = axis;
+        B.cx = centre[0]; B.cy = centre[1]; B.cz = centre[2];
+        B.r = r;
+        B.mpb = cfg.machine_precision_buffer;
         const int layer = 2;
         if (B.type == NS_BODY_SPHERE) B.filterWidth = layer * std::max({h[0], h[1], h[2]});
         else if (B.axis == 0) B.filterWidth = layer * std::max(h[1], h[2]);
@@ -242,41 +253,83 @@ void Case::setupBoundaries() {
             lo[a] = (int)(ci - rad);
             hi[a] = (int)(ci + rad);
         }
+        B.box = Window{lo[0], lo[1], lo[2], hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+        for (int q = 0; q < bodies.n; ++q) {
+            const Window& o = bodies.b[q].box;
+            bool apart = false;
+            for (int a = 0; a < 3; ++a) {
+                const int l0 = a == 0 ? o.i0 : a == 1 ? o.j0 : o.k0, n0 = a == 0 ? o.ni : a == 1 ? o.nj : o.nk;
+                apart = apart || hi[a] < l0 || l0 + n0 - 1 < lo[a];
+            }
+            if (!apart) throw GeometryError("the bounding boxes of two immersed bodies overlap");
+        }
+        bodies.b[bodies.n++] = B;
+    };
+    if (cfg.body_type != NS_BODY_NONE) addBody(cfg.body_type, cfg.body_axis, cfg.body_centre, cfg.body_radius);
+    for (int q = 0; q < cfg.n_extra_bodies; ++q)
+        addBody(cfg.extra_bodies[q].type, cfg.extra_bodies[q].axis, cfg.extra_bodies[q].centre, cfg.extra_bodies[q].radius);
+    // the window: the smallest box holding every body's box (side tables and ghost map are dense over it)
+    if (bodies.n > 0) {
+        int lo[3] = {Ns[0], Ns[1], Ns[2]}, hi[3] = {-1, -1, -1};
+        for (int q = 0; q < bodies.n; ++q) {
+            const Window& o = bodies.b[q].box;
+            const int l[3] = {o.i0, o.j0, o.k0}, n[3] = {o.ni, o.nj, o.nk};
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], l[a]);
+                hi[a] = std::max(hi[a], l[a] + n[a] - 1);
+            }
+        }
         W = Window{lo[0], lo[1], lo[2], hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
-        B.box = W;
     }
 }
 
-// Mesh::categorizeNodes, src/Mesh.cpp:61-82, on the device.
+// Mesh::categorizeNodes, src/Mesh.cpp:61-82, on the device. The bodies' boxes are disjoint, so classifying
+// them in one pass equals the reference's body-by-body classification; ghost augmentation, ghost and filter
+// lists run body by body, each over its own box, so every list holds one body's entries after the previous
+// body's.
 void Case::categorizeNodes() {
     cls = (uint8_t*)alloc((size_t)L.NK * L.plane);
     hipCheck(hipMemsetAsync(cls, 0, (size_t)L.NK * L.plane, stream), "memset cls");
     int ft[6];
     for (int q = 0; q < 6; ++q) ft[q] = (cfg.bc[q] == NS_BC_INLET || cfg.bc[q] == NS_BC_OUTLET) ? NS_FLUID_EDGE : NS_FLUID_GHOST;
-    launchClassify(stream, L, cls, ft, B, P);
+    launchClassify(stream, L, cls, ft, bodies, P);
     nGhost = nFilter = nInactive = 0;
-    if (B.type == NS_BODY_NONE) return;
+    for (int q = 0; q <= NS_MAX_BODIES; ++q) ghostOff[q] = filterOff[q] = 0;
+    if (bodies.n == 0) return;
     launchMarkGhosts(stream, L, cls, W);
-    size_t nb = (W.size() + 255) / 256;
-    int32_t* counts = (int32_t*)alloc(nb * sizeof(int32_t));
-    int32_t* offsets = (int32_t*)alloc(nb * sizeof(int32_t));
-    int32_t* flags = (int32_t*)alloc(2 * sizeof(int32_t));  // changed, error
-    int64_t* list = (int64_t*)alloc(W.size() * sizeof(int64_t));
-    double* bi = (double*)alloc(3 * W.size() * sizeof(double));
-    double* ip = (double*)alloc(3 * W.size() * sizeof(double));
-    // iterative ghost augmentation (CylinderBody::identifyRelatedNodes, src/Boundary.cpp:907-916): repeat
-    // until no SolidInactive node remains in any image-point box
-    for (int round = 0;; ++round) {
-        int64_t n = compactWindow(stream, L, cls, W, CLS_TYPE_MASK, NS_SOLID_GHOST, list, counts, offsets);
-        hipCheck(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), stream), "memset flags");
-        launchImagePoints(stream, L, cls, P, B, list, n, bi, ip, flags, flags + 1);
-        int32_t h[2];
-        hipCheck(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, stream), "copy flags");
-        hipCheck(hipStreamSynchronize(stream), "sync");
-        if (h[1]) throw GeometryError("image point box leaves the mesh");
-        if (h[0] == 0) { nGhost = n; break; }
-        if (round > 1000) throw GeometryError("ghost augmentation does not converge");
+    size_t cap = 0, nbMax = 0;  // list capacity: the sum of the body boxes
+    for (int b = 0; b < bodies.n; ++b) {
+        cap += bodies.b[b].box.size();
+        nbMax = std::max(nbMax, (bodies.b[b].box.size() + 255) / 256);
     }
+    int32_t* counts = (int32_t*)alloc(nbMax * sizeof(int32_t));
+    int32_t* offsets = (int32_t*)alloc(nbMax * sizeof(int32_t));
+    int32_t* flags = (int32_t*)alloc(2 * sizeof(int32_t));  // changed, error
+    int64_t* list = (int64_t*)alloc(cap * sizeof(int64_t));
+    double* bi = (double*)alloc(3 * cap * sizeof(double));
+    double* ip = (double*)alloc(3 * cap * sizeof(double));
+    auto readFlags = [&](int32_t* h) {
+        hipCheck(hipMemcpyAsync(h, flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, stream), "copy flags");
+        hipCheck(hipStreamSynchronize(stream), "sync");
+    };
+    // iterative ghost augmentation (CylinderBody::identifyRelatedNodes, src/Boundary.cpp:907-916), body by
+    // body: repeat until no SolidInactive node remains in any of the body's image-point boxes
+    for (int b = 0; b < bodies.n; ++b) {
+        const BodyDesc& B = bodies.b[b];
+        const int64_t base = ghostOff[b];
+        for (int round = 0;; ++round) {
+            int64_t n = compactWindow(stream, L, cls, B.box, CLS_TYPE_MASK, NS_SOLID_GHOST, list + base, counts, offsets);
+            hipCheck(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), stream), "memset flags");
+            launchImagePoints(stream, L, cls, P, B, list + base, n, bi + 3 * base, ip + 3 * base, flags, flags + 1);
+            int32_t h[2];
+            readFlags(h);
+            if (h[1] & 2) throw GeometryError("an image-point box of one immersed body reaches another body's solid nodes");
+            if (h[1]) throw GeometryError("image point box leaves the mesh");
+            if (h[0] == 0) { ghostOff[b + 1] = base + n; break; }
+            if (round > 1000) throw GeometryError("ghost augmentation does not converge");
+        }
+    }
+    nGhost = ghostOff[bodies.n];
     ghostList = list;
     int32_t* gmap = (int32_t*)alloc(W.size() * sizeof(int32_t));
     hipCheck(hipMemsetAsync(gmap, 0xff, W.size() * sizeof(int32_t), stream), "memset gmap");
@@ -292,22 +345,151 @@ void Case::categorizeNodes() {
     G.perm = (int8_t*)alloc(32 * nGhost);
     G.boxLo = (int32_t*)alloc(3 * nGhost * sizeof(int32_t));
     hipCheck(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), stream), "memset flags");
-    launchGhostGeometry(stream, L, cls, P, W, ghostList, nGhost, gmap, G, flags + 1);
+    for (int b = 0; b < bodies.n; ++b)
+        launchGhostGeometry(stream, L, cls, P, W, ghostList + ghostOff[b], ghostOff[b], ghostOff[b + 1] - ghostOff[b],
+                            gmap, ghostTablesFrom(G, ghostOff[b]), flags + 1);
     int32_t h[2];
-    hipCheck(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, stream), "copy flags");
-    hipCheck(hipStreamSynchronize(stream), "sync");
+    readFlags(h);
     if (h[1] & 4) throw GeometryError("Impossible situation. Found solid node around image point.");
+    if (h[1] & 2) throw GeometryError("an image-point box holds a ghost node of another immersed body");
     if (h[1]) throw GeometryError("ghost geometry error");
-    // filter nodes (ascending)
-    filterList = (int64_t*)alloc(std::max<int64_t>(1, (int64_t)W.size()) * sizeof(int64_t));
-    nFilter = compactWindow(stream, L, cls, W, CLS_FILTER, CLS_FILTER, filterList, counts, offsets);
+    // filter nodes (ascending within each body)
+    filterList = (int64_t*)alloc(std::max<size_t>(1, cap) * sizeof(int64_t));
+    for (int b = 0; b < bodies.n; ++b)
+        filterOff[b + 1] = filterOff[b] + compactWindow(stream, L, cls, bodies.b[b].box, CLS_FILTER, CLS_FILTER,
+                                                        filterList + filterOff[b], counts, offsets);
+    nFilter = filterOff[bodies.n];
     // SolidInactive bookkeeping for the dt scan
+    size_t nbW = (W.size() + 255) / 256;
+    int32_t* countsW = nbW > nbMax ? (int32_t*)alloc(nbW * sizeof(int32_t)) : counts;
+    int32_t* offsetsW = nbW > nbMax ? (int32_t*)alloc(nbW * sizeof(int32_t)) : offsets;
     dim3 blk(64), grd(nblk(W.ni, 64), W.nj, W.nk);
     hipLaunchKernelGGL(k_flag_inactive, grd, blk, 0, stream, L, cls, W, cfg.bc[NS_XMIN] == NS_BC_PERIODIC,
                        cfg.bc[NS_YMIN] == NS_BC_PERIODIC, cfg.bc[NS_ZMIN] == NS_BC_PERIODIC);
     inactiveList = (int64_t*)alloc(W.size() * sizeof(int64_t));
-    nInactive = compactWindow(stream, L, cls, W, CLS_INACTIVE_IC, CLS_INACTIVE_IC, inactiveList, counts, offsets);
+    nInactive = compactWindow(stream, L, cls, W, CLS_INACTIVE_IC, CLS_INACTIVE_IC, inactiveList, countsW, offsetsW);
     inactiveIC = (double*)alloc(5 * std::max<int64_t>(1, nInactive) * sizeof(double));
+    findFixedBoundary();
+}
+
+// The reference's symmetry face mirrors the stored primitive values of the node two in
+// (src/Boundary.cpp:336-417); a periodic face copies every stored value of its source
+// (src/Boundary.cpp:250-300). An untouched SolidInactive node keeps the primitive values of the initial
+// state for good, while its conserved values alternate between the initial state and the zero-initialised
+// second array (the Old/Cur swap, src/Mesh.cpp:144-151). A symmetry node whose source chain ends at such a
+// node therefore holds constant conserved values, which the forms that derive primitives from the state
+// cannot reproduce: they are found here, once, and written after every edge-face pass.
+void Case::findFixedBoundary() {
+    nFixed = 0;
+    fixedHostNodes.clear();
+    fixedSrc.clear();
+    fixedMirror.clear();
+    bool sym = false;
+    for (int q = 0; q < 6; ++q) sym = sym || cfg.bc[q] == NS_BC_SYMMETRY;
+    if (bodies.n == 0 || !sym || nInactive == 0) return;
+    // a chain reaches at most three nodes in from a face: only windows that come that close matter
+    const int Nn[3] = {L.NI, L.NJ, L.NK}, wl[3] = {W.i0, W.j0, W.k0}, wn[3] = {W.ni, W.nj, W.nk};
+    bool near = false;
+    for (int a = 0; a < 3; ++a) near = near || wl[a] <= 3 || wl[a] + wn[a] - 1 >= Nn[a] - 4;
+    if (!near) return;
+    std::vector<uint8_t> hc((size_t)L.NK * L.plane);
+    hipCheck(hipMemcpyAsync(hc.data(), cls, hc.size(), hipMemcpyDeviceToHost, stream), "copy cls");
+    hipCheck(hipStreamSynchronize(stream), "sync");
+    auto owner = [&](int i, int j, int k) -> const EdgeFace* {
+        for (int q = 0; q < 6; ++q) {
+            const EdgeFace& f = faces[q];
+            if (i >= f.i0 && i <= f.i1 && j >= f.j0 && j <= f.j1 && k >= f.k0 && k <= f.k1) return &f;
+        }
+        return nullptr;
+    };
+    struct Src { bool ok; int i, j, k; uint8_t mirror; };
+    // where the stored primitives of (i,j,k) come from, when they are constant
+    std::function<Src(int, int, int, int)> prims = [&](int i, int j, int k, int depth) -> Src {
+        if (depth > 6) return Src{false, 0, 0, 0, 0};
+        if (hc[L.off(i, j, k)] & CLS_INACTIVE_IC) return Src{true, i, j, k, 0};
+        const EdgeFace* f = owner(i, j, k);
+        if (!f) return Src{false, 0, 0, 0, 0};
+        int c[3] = {i, j, k};
+        if (f->type == NS_BC_SYMMETRY) {
+            c[f->axis] += f->side == 0 ? 2 : -2;
+            Src r = prims(c[0], c[1], c[2], depth + 1);
+            if (r.ok) r.mirror ^= (uint8_t)(1u << f->axis);
+            return r;
+        }
+        if (f->type == NS_BC_PERIODIC) {
+            c[f->axis] = f->side == 0 ? Nn[f->axis] - 2 : 1;
+            return prims(c[0], c[1], c[2], depth + 1);
+        }
+        return Src{false, 0, 0, 0, 0};
+    };
+    // nodes with constant conserved values: symmetry nodes with constant primitives, and periodic copies
+    // of such nodes
+    std::function<Src(int, int, int, int)> fixedCons = [&](int i, int j, int k, int depth) -> Src {
+        if (depth > 6) return Src{false, 0, 0, 0, 0};
+        const EdgeFace* f = owner(i, j, k);
+        if (!f) return Src{false, 0, 0, 0, 0};
+        if (f->type == NS_BC_SYMMETRY) return prims(i, j, k, depth);
+        if (f->type == NS_BC_PERIODIC) {
+            int c[3] = {i, j, k};
+            c[f->axis] = f->side == 0 ? Nn[f->axis] - 2 : 1;
+            return fixedCons(c[0], c[1], c[2], depth + 1);
+        }
+        return Src{false, 0, 0, 0, 0};
+    };
+    auto visit = [&](int i, int j, int k) {
+        Src r = fixedCons(i, j, k, 0);
+        if (!r.ok) return;
+        fixedHostNodes.push_back((int64_t)L.off(i, j, k));
+        fixedSrc.push_back((int64_t)L.off(r.i, r.j, r.k));
+        fixedMirror.push_back(r.mirror);
+    };
+    for (int k = 0; k < L.NK; ++k)
+        for (int j = 0; j < L.NJ; ++j) {
+            const bool edgeRow = k == 0 || k == L.NK - 1 || j == 0 || j == L.NJ - 1;
+            if (edgeRow) {
+                for (int i = 0; i < L.NI; ++i) visit(i, j, k);
+            } else {
+                visit(0, j, k);
+                visit(L.NI - 1, j, k);
+            }
+        }
+    nFixed = (int64_t)fixedHostNodes.size();
+    if (nFixed == 0) return;
+    fixedNodes = (int64_t*)alloc(nFixed * sizeof(int64_t));
+    fixedVals = (double*)alloc(5 * nFixed * sizeof(double));
+    hipCheck(hipMemcpyAsync(fixedNodes, fixedHostNodes.data(), nFixed * sizeof(int64_t), hipMemcpyHostToDevice, stream),
+             "copy fixed nodes");
+    hipCheck(hipStreamSynchronize(stream), "sync");
+}
+
+// Values of the constant boundary nodes from the state being uploaded (its primitive values are the stored
+// ones of the untouched solid nodes): the source's primitives, mirrored, turned into conserved values as
+// the symmetry face does (src/FlowVariableGroupStructs.h:152-185).
+template <class T>
+void Case::computeFixedBoundary(const std::vector<T>& h) {
+    if (nFixed == 0) return;
+    const double g = cfg.gamma;
+    std::vector<double> v(5 * nFixed);
+    for (int64_t q = 0; q < nFixed; ++q) {
+        const size_t o = (size_t)fixedSrc[q];
+        const double rho = (double)h[o], ru = (double)h[o + L.fstride], rv = (double)h[o + 2 * L.fstride],
+                     rw = (double)h[o + 3 * L.fstride], rE = (double)h[o + 4 * L.fstride];
+        double u = ru / (1 + rho), vv = rv / (1 + rho), w = rw / (1 + rho);
+        const double p = (g - 1) * (rE - (1 + rho) / 2 * (u * u + vv * vv + w * w));
+        const double Tt = (g * p - rho) / (1 + rho);
+        if (fixedMirror[q] & 1) u = -u;
+        if (fixedMirror[q] & 2) vv = -vv;
+        if (fixedMirror[q] & 4) w = -w;
+        const double r2 = (g * p - Tt) / (1 + Tt);
+        v[5 * q] = r2;
+        v[5 * q + 1] = (1 + r2) * u;
+        v[5 * q + 2] = (1 + r2) * vv;
+        v[5 * q + 3] = (1 + r2) * w;
+        v[5 * q + 4] = p / (g - 1) + (1 + r2) / 2 * (u * u + vv * vv + w * w);
+    }
+    hipCheck(hipMemcpyAsync(fixedVals, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, stream),
+             "copy fixed values");
+    hipCheck(hipStreamSynchronize(stream), "sync");
 }
 
 template <class T>
@@ -398,6 +580,7 @@ void Case::hostInitialField(std::vector<T>& h) const {
 
 template <class T>
 void Case::uploadState(const std::vector<T>& h) {
+    computeFixedBoundary(h);
     hipCheck(hipMemcpyAsync(U, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream), "upload IC");
     if (comp) hipCheck(hipMemsetAsync(comp, 0, 5 * L.fstride * sizeof(int8_t), stream), "memset");
     // pre-filter side tables start as the current values (the reference's primitives match the IC)
@@ -446,10 +629,16 @@ void Case::applyBCs(T* S, double frac, bool outletDone, T* storedPrims) {
     const double* tdev = &scal->t;
     const double* dtdev = &scal->dt;
     launchEdgeFaces<T>(stream, L, P, outletDone ? bcFaces : faces, S, tdev, dtdev, frac, storedPrims);
-    if (B.type == NS_BODY_NONE) return;
-    launchFilterCompute<T>(stream, L, W, filterList, nFilter, S, (T*)ftmp, (T*)rhoPre, (T*)rhoEPre);
-    launchFilterWrite<T>(stream, L, filterList, nFilter, S, (const T*)ftmp);
-    launchGhostApply<T>(stream, L, P, G, nGhost, S, storedPrims);
+    if (!storedPrims && nFixed > 0)  // the stored-primitive form mirrors the stored values itself
+        hipLaunchKernelGGL(k_apply_fixed<T>, dim3(nblk(nFixed, 128)), dim3(128), 0, stream, L, (const int64_t*)fixedNodes,
+                           (const double*)fixedVals, nFixed, S);
+    // ImmersedBoundary::applyBoundaryCondition body by body, in list order (src/Mesh.cpp:165-166)
+    for (int b = 0; b < bodies.n; ++b) {
+        const int64_t f0 = filterOff[b], nf = filterOff[b + 1] - f0, g0 = ghostOff[b], ng = ghostOff[b + 1] - g0;
+        launchFilterCompute<T>(stream, L, W, filterList + f0, nf, S, (T*)ftmp, (T*)rhoPre, (T*)rhoEPre);
+        launchFilterWrite<T>(stream, L, filterList + f0, nf, S, (const T*)ftmp);
+        launchGhostApply<T>(stream, L, P, ghostTablesFrom(G, g0), ng, S, storedPrims);
+    }
 }
 
 // Solver::marchTimeStep, src/Solver.cpp:130-167 (without the rho filter, see ns_init)
@@ -917,13 +1106,15 @@ void Case::getField(int which, double* out) {
 namespace {
 
 constexpr uint64_t kCkptMagic = 0x31544b5049534e4eULL;  // "NNSIPKT1"
-constexpr uint32_t kCkptVersion = 1;
+constexpr uint32_t kCkptVersion = 2;
 
 // Configuration fields a checkpoint must agree on: the grid, boundaries, body, initial-state generator
 // and physics (stopping, output and stability-limit settings may change between runs).
 struct CkptKey {
     int32_t NI, NJ, NK, precision, kahan, bc[6], body_type, body_axis;
     double body_centre[3], body_radius, gamma, Pr, Re, sutherlands_C2, M_0, T_0, L_x, L_y, L_z, inlet_ramp_time;
+    int32_t n_extra_bodies, pad;
+    ns_body extra_bodies[NS_MAX_BODIES - 1];
 };
 CkptKey ckptKey(const ns_config& c) {
     CkptKey k;
@@ -935,6 +1126,11 @@ CkptKey ckptKey(const ns_config& c) {
     for (int q = 0; q < 3; ++q) k.body_centre[q] = c.body_centre[q];
     k.body_radius = c.body_radius; k.gamma = c.gamma; k.Pr = c.Pr; k.Re = c.Re; k.sutherlands_C2 = c.sutherlands_C2;
     k.M_0 = c.M_0; k.T_0 = c.T_0; k.L_x = c.L_x; k.L_y = c.L_y; k.L_z = c.L_z; k.inlet_ramp_time = c.inlet_ramp_time;
+    k.n_extra_bodies = c.n_extra_bodies;
+    for (int q = 0; q < c.n_extra_bodies && q < NS_MAX_BODIES - 1; ++q) {
+        k.extra_bodies[q] = c.extra_bodies[q];
+        if (c.extra_bodies[q].type != NS_BODY_CYLINDER) k.extra_bodies[q].axis = 0;
+    }
     return k;
 }
 

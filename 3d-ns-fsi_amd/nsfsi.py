@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnsfsi.so")
 
 NS_MAX_FILTER = 16
+NS_MAX_BODIES = 8
 
 STATUS = {
     0: "NS_OK", 1: "NS_ERR_CONFIG", 2: "NS_ERR_GEOMETRY", 3: "NS_ERR_NONFINITE", 4: "NS_ERR_HIP",
@@ -28,6 +29,21 @@ BODY = {"none": 0, "cylinder": 1, "sphere": 2}
 IC = {"stagnation": 0, "uniform_oblique": 1, "freestream_gaussian": 2, "freestream_uniform": 3}
 FIELDS = ["rho", "rho_u", "rho_v", "rho_w", "rho_E", "u", "v", "w", "p", "T", "mu", "kappa"]
 NODE_TYPES = ["FluidInterior", "FluidEdge", "FluidGhost", "SolidInactive", "SolidGhost"]
+
+
+class NSBody(C.Structure):
+    """ctypes image of ``ns_body`` (include/ns_fsi.h)."""
+
+    _fields_ = [("type", C.c_int32), ("axis", C.c_int32), ("centre", C.c_double * 3), ("radius", C.c_double)]
+
+    def _key(self):
+        return self.type, self.axis, tuple(self.centre), self.radius
+
+    def __eq__(self, other):
+        return isinstance(other, NSBody) and self._key() == other._key()
+
+    def __hash__(self):
+        return hash(self._key())
 
 
 class NSConfig(C.Structure):
@@ -66,6 +82,9 @@ class NSConfig(C.Structure):
         ("inlet_ramp_time", C.c_double),
         ("precision", C.c_int32),
         ("kahan", C.c_int32),
+        ("n_extra_bodies", C.c_int32),
+        ("pad_bodies", C.c_int32),
+        ("extra_bodies", NSBody * (NS_MAX_BODIES - 1)),
     ]
 
 
@@ -168,7 +187,8 @@ def make_config(**kw) -> NSConfig:
 
     Names follow ns_config; ``bc`` takes 6 names (x-min .. z-max), ``body_type``/``ic_type`` take names,
     ``filter_intervals``/``filter_change_times`` take lists, ``body_centre`` a 3-sequence,
-    ``precision`` "fp64"/"fp32".
+    ``precision`` "fp64"/"fp32", ``extra_bodies`` a list of dicts (type, axis, centre, radius) for the bodies
+    after the first.
     """
     c = default_config()
     for k, v in kw.items():
@@ -192,6 +212,18 @@ def make_config(**kw) -> NSConfig:
             c.n_filter_change_times = len(v)
             for q, x in enumerate(v):
                 c.filter_change_times[q] = float(x)
+        elif k == "extra_bodies":
+            if len(v) > NS_MAX_BODIES - 1:
+                raise ValueError(f"at most {NS_MAX_BODIES - 1} extra bodies")
+            c.n_extra_bodies = len(v)
+            for q, d in enumerate(v):
+                b = c.extra_bodies[q]
+                b.type = BODY[d["type"]] if isinstance(d["type"], str) else int(d["type"])
+                ax = d.get("axis", 0)
+                b.axis = "xyz".index(ax) if isinstance(ax, str) else int(ax)
+                for a in range(3):
+                    b.centre[a] = float(d["centre"][a])
+                b.radius = float(d["radius"])
         elif k == "precision":
             c.precision = {"fp64": 0, "fp32": 1}[v] if isinstance(v, str) else int(v)
         elif k == "stop_criterion":
@@ -246,6 +278,13 @@ def config_text(c: NSConfig) -> str:
         f"inletRampTime = {f(c.inlet_ramp_time)}", f'precision = "{"fp64" if c.precision == 0 else "fp32"}"',
         f"kahan = {b(c.kahan)}",
     ]
+    if c.n_extra_bodies:
+        ent = []
+        for q in range(c.n_extra_bodies):
+            e = c.extra_bodies[q]
+            ent.append(f'{{ type = "{inv_body[e.type]}"; axis = "{"xyz"[e.axis]}"; centre = ['
+                       + ", ".join(f(e.centre[a]) for a in range(3)) + f"]; radius = {f(e.radius)}; }}")
+        lines.append("extraBodies = (" + ",\n    ".join(ent) + ")")
     return "\n".join(lines) + "\n"
 
 

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define NS_FSI_ABI_VERSION 1
+#define NS_FSI_ABI_VERSION 2
 
 typedef enum ns_status {
     NS_OK = 0,
@@ -63,6 +63,15 @@ typedef enum ns_node_type {
 } ns_node_type;
 
 #define NS_MAX_FILTER 16
+#define NS_MAX_BODIES 8
+
+/* One immersed body (the reference's CylinderBody / SphereBody, src/Boundary.h:255-346). */
+typedef struct ns_body {
+    int32_t type;       /* ns_body_type */
+    int32_t axis;       /* 0,1,2 = x,y,z (cylinder only) */
+    double centre[3];
+    double radius;
+} ns_body;
 
 /* Every ConfigFile key of the reference (ConfigFile:22-125, src/ConfigSettings.h:39-74) plus the
  * extension keys for what the reference hard-codes (src/Mesh.cpp:33-57, src/Solver.cpp:26,
@@ -99,6 +108,12 @@ typedef struct ns_config {
     double inlet_ramp_time;            /* reference: 10 (src/Boundary.cpp:267) */
     int32_t precision;                 /* ns_precision */
     int32_t kahan;                     /* FP32 only: compensated stage combine */
+    /* Further immersed bodies after the one above, classified and applied in this order as the reference's
+     * vector of bodies (src/Mesh.cpp:52-57, src/Mesh.h:91-92). Their bounding boxes (radius + 3 nodes) must be
+     * disjoint from each other's. */
+    int32_t n_extra_bodies;
+    int32_t pad_bodies;
+    ns_body extra_bodies[NS_MAX_BODIES - 1];
 } ns_config;
 
 typedef struct ns_handle ns_handle;

@@ -200,34 +200,65 @@ public:
                 for (size_t kk = e.owned.kMin; kk <= e.owned.kMax; ++kk) nodeType[id(i, j, kk)] = t;
     }
 
-    // ---------------------------------------------------------------- immersed body
+    // ---------------------------------------------------------------- immersed bodies
+    // One ImmersedBoundary each (src/Boundary.h:171-193): its own ghost list, ghost map and filter nodes,
+    // classified and applied in the order of the body list (src/Mesh.cpp:52-57, :73-74, :165-166).
+    struct Body {
+        int type, axis;
+        V3 c;
+        double r;
+        std::vector<Ghost> ghosts;
+        std::map<size_t, size_t> ghostMap;
+        std::vector<size_t> filterNodes;
+    };
+    std::vector<Body> bodies;
+    // every body's lists concatenated in body order (what the inspection calls report)
     std::vector<Ghost> ghosts;
-    std::map<size_t, size_t> ghostMap;
     std::vector<size_t> filterNodes;
 
-    V3 centre() const { return {P.body_centre[0], P.body_centre[1], P.body_centre[2]}; }
+    void setupBodies() {
+        bodies.clear();
+        if (P.n_extra_bodies < 0 || P.n_extra_bodies > NS_MAX_BODIES - 1)
+            throw std::invalid_argument("n_extra_bodies out of range");
+        if (P.n_extra_bodies > 0 && P.body_type == NS_BODY_NONE)
+            throw std::invalid_argument("extra bodies need a first body");
+        auto add = [&](int type, int axis, const double* c, double r) {
+            if (type != NS_BODY_CYLINDER && type != NS_BODY_SPHERE) throw std::invalid_argument("body type");
+            if (type == NS_BODY_CYLINDER && (axis < 0 || axis > 2)) throw std::invalid_argument("cylinder axis");
+            if (!(r > 0)) throw std::invalid_argument("body radius must be positive");
+            Body b;
+            b.type = type;
+            b.axis = axis;
+            b.c = {c[0], c[1], c[2]};
+            b.r = r;
+            bodies.push_back(b);
+        };
+        if (P.body_type != NS_BODY_NONE) add(P.body_type, P.body_axis, P.body_centre, P.body_radius);
+        for (int q = 0; q < P.n_extra_bodies; ++q)
+            add(P.extra_bodies[q].type, P.extra_bodies[q].axis, P.extra_bodies[q].centre, P.extra_bodies[q].radius);
+    }
 
     // CylinderBody::getNormalProbe / SphereBody::getNormalProbe, src/Boundary.cpp:920-935, :1047-1053
-    V3 normalProbe(V3 g) const {
-        V3 c2g = g - centre();
-        if (P.body_type == NS_BODY_CYLINDER) {
-            if (P.body_axis == 0) c2g.x = 0; else if (P.body_axis == 1) c2g.y = 0; else c2g.z = 0;
+    static V3 normalProbe(const Body& B, V3 g) {
+        V3 c2g = g - B.c;
+        if (B.type == NS_BODY_CYLINDER) {
+            if (B.axis == 0) c2g.x = 0; else if (B.axis == 1) c2g.y = 0; else c2g.z = 0;
         }
-        double lf = (P.body_radius - length(c2g)) / length(c2g);
+        double lf = (B.r - length(c2g)) / length(c2g);
         return c2g * lf;
     }
 
     // getCylinderBoundingBox / getSphereBoundingBox, src/Boundary.cpp:938-965, :1056-1072
-    Box bodyBox(size_t layer) const {
-        double r = P.body_radius;
+    Box bodyBox(const Body& B, size_t layer) const {
+        double r = B.r;
         long rad[3] = {(long)std::ceil(r / h.x) + 1 + (long)layer, (long)std::ceil(r / h.y) + 1 + (long)layer,
                        (long)std::ceil(r / h.z) + 1 + (long)layer};
-        double c[3] = {P.body_centre[0], P.body_centre[1], P.body_centre[2]};
+        double c[3] = {B.c.x, B.c.y, B.c.z};
         double hh[3] = {h.x, h.y, h.z};
         long lo[3], hi[3];
         long Nn[3] = {(long)NI, (long)NJ, (long)NK};
         for (int a = 0; a < 3; ++a) {
-            if (P.body_type == NS_BODY_CYLINDER && a == P.body_axis) { lo[a] = 0; hi[a] = Nn[a] - 1; continue; }
+            if (B.type == NS_BODY_CYLINDER && a == B.axis) { lo[a] = 0; hi[a] = Nn[a] - 1; continue; }
             double rc = std::round(c[a] / hh[a]);  // centre index without the origin offset (reference quirk)
             if (!(rc >= 0)) throw GeometryError("body bounding box below index 0");
             long ci = (long)rc;
@@ -239,34 +270,34 @@ public:
     }
 
     // getSolidAndFilterNodesIn{Cylinder,Sphere}, src/Boundary.cpp:968-1014, :1075-1103
-    void solidAndFilter(const Box& b, size_t layer, std::vector<size_t>& solids) {
-        V3 c = centre();
+    void solidAndFilter(Body& B, const Box& b, size_t layer, std::vector<size_t>& solids) {
+        V3 c = B.c;
         double fw;
-        if (P.body_type == NS_BODY_SPHERE) fw = layer * std::max({h.x, h.y, h.z});
-        else if (P.body_axis == 0) fw = layer * std::max(h.y, h.z);
-        else if (P.body_axis == 1) fw = layer * std::max(h.x, h.z);
+        if (B.type == NS_BODY_SPHERE) fw = layer * std::max({h.x, h.y, h.z});
+        else if (B.axis == 0) fw = layer * std::max(h.y, h.z);
+        else if (B.axis == 1) fw = layer * std::max(h.x, h.z);
         else fw = layer * std::max(h.x, h.y);
         for (size_t i = b.iMin; i <= b.iMax; ++i)
             for (size_t j = b.jMin; j <= b.jMax; ++j)
                 for (size_t kk = b.kMin; kk <= b.kMax; ++kk) {
                     V3 p = pos(i, j, kk);
                     double dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z, d;
-                    if (P.body_type == NS_BODY_SPHERE) d = std::sqrt(dx * dx + dy * dy + dz * dz);
-                    else if (P.body_axis == 0) d = std::sqrt(dy * dy + dz * dz);
-                    else if (P.body_axis == 1) d = std::sqrt(dx * dx + dz * dz);
+                    if (B.type == NS_BODY_SPHERE) d = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    else if (B.axis == 0) d = std::sqrt(dy * dy + dz * dz);
+                    else if (B.axis == 1) d = std::sqrt(dx * dx + dz * dz);
                     else d = std::sqrt(dx * dx + dy * dy);
                     size_t n = id(i, j, kk);
-                    if (d < P.body_radius - P.machine_precision_buffer) {
+                    if (d < B.r - P.machine_precision_buffer) {
                         nodeType[n] = SolidInactive;
                         solids.push_back(n);
-                    } else if (d < P.body_radius + fw && nodeType[n] == FluidInterior) {
-                        filterNodes.push_back(n);
+                    } else if (d < B.r + fw && nodeType[n] == FluidInterior) {
+                        B.filterNodes.push_back(n);
                     }
                 }
     }
 
     // findGhostNodesWithFluidNeighbors, src/Boundary.cpp:461-562: 18 neighbours (no corners)
-    void findGhosts(const std::vector<size_t>& solids) {
+    void findGhosts(Body& B, const std::vector<size_t>& solids) {
         for (size_t n : solids) {
             U3 s = ijk(n);
             bool has = false;
@@ -280,20 +311,20 @@ public:
                         if (nodeType[id(ii, jj, kk)] == FluidInterior) has = true;
                     }
             if (has) {
-                ghostMap[n] = ghosts.size();
-                ghosts.push_back({s, {}, {}});
+                B.ghostMap[n] = B.ghosts.size();
+                B.ghosts.push_back({s, {}, {}});
                 nodeType[n] = SolidGhost;
             }
         }
     }
 
     // setImagePointPositions + checkIfSurroundingShouldBeGhost, src/Boundary.cpp:568-602
-    std::vector<Ghost> imagePoints(size_t first) {
+    std::vector<Ghost> imagePoints(Body& B, size_t first) {
         std::vector<Ghost> fresh;
-        for (size_t g = first; g < ghosts.size(); ++g) {
-            Ghost& gh = ghosts[g];
+        for (size_t g = first; g < B.ghosts.size(); ++g) {
+            Ghost& gh = B.ghosts[g];
             V3 gp = pos(gh.idx.i, gh.idx.j, gh.idx.k);
-            V3 pr = normalProbe(gp);
+            V3 pr = normalProbe(B, gp);
             gh.bi = gp + pr;
             gh.ip = gh.bi + pr;
             Box b = surrounding(gh.ip);
@@ -312,21 +343,27 @@ public:
         Box unclaimed{0, NI - 1, 0, NJ - 1, 0, NK - 1};
         for (auto& e : edges) claim(e, unclaimed);
         std::reverse(edges.begin(), edges.end());
-        if (P.body_type != NS_BODY_NONE) {
+        for (Body& B : bodies) {
             const size_t layer = 2;
-            Box b = bodyBox(layer);
+            Box b = bodyBox(B, layer);
             std::vector<size_t> solids;
-            solidAndFilter(b, layer, solids);
-            findGhosts(solids);
-            std::vector<Ghost> fresh = imagePoints(0);
+            solidAndFilter(B, b, layer, solids);
+            findGhosts(B, solids);
+            std::vector<Ghost> fresh = imagePoints(B, 0);
             while (!fresh.empty()) {
-                size_t first = ghosts.size();
+                size_t first = B.ghosts.size();
                 for (auto& g : fresh) {  // appendGhostNodes, src/Boundary.cpp:605-615
-                    ghostMap[id(g.idx.i, g.idx.j, g.idx.k)] = ghosts.size();
-                    ghosts.push_back(g);
+                    B.ghostMap[id(g.idx.i, g.idx.j, g.idx.k)] = B.ghosts.size();
+                    B.ghosts.push_back(g);
                 }
-                fresh = imagePoints(first);
+                fresh = imagePoints(B, first);
             }
+        }
+        ghosts.clear();
+        filterNodes.clear();
+        for (const Body& B : bodies) {
+            ghosts.insert(ghosts.end(), B.ghosts.begin(), B.ghosts.end());
+            filterNodes.insert(filterNodes.end(), B.filterNodes.begin(), B.filterNodes.end());
         }
         interior.clear();
         solidGhost.clear();
@@ -667,7 +704,8 @@ public:
     }
 
     // ---------------------------------------------------------------- immersed BC, src/Boundary.cpp:427-457, 618-883
-    void applyImmersed() {
+    void applyImmersed(const Body& B) {
+        const std::vector<size_t>& filterNodes = B.filterNodes;
         // filterClosestFluidNodes (two-phase), src/Boundary.cpp:618-652
         std::vector<double> fr, fe;
         fr.reserve(filterNodes.size());
@@ -680,7 +718,7 @@ public:
             fe.push_back((6 * E[n] + E[n + sI] + E[n - sI] + E[n + sJ] + E[n - sJ] + E[n + 1] + E[n - 1]) / 12);
         }
         for (size_t q = 0; q < filterNodes.size(); ++q) { cons[0][filterNodes[q]] = fr[q]; cons[4][filterNodes[q]] = fe[q]; }
-        for (const Ghost& g : ghosts) {
+        for (const Ghost& g : B.ghosts) {
             Box b = surrounding(g.ip);
             double val[5][8], X[8], Y[8], Z[8];
             bool gflag[8] = {false, false, false, false, false, false, false, false};
@@ -698,7 +736,10 @@ public:
                 } else if (t == SolidGhost) {
                     gflag[c] = true;
                     allFluid = false;
-                    const Ghost& o = ghosts[ghostMap.at(n)];
+                    // ghostNodeMap.at (src/Boundary.cpp:681) finds only this body's ghosts
+                    auto it = B.ghostMap.find(n);
+                    if (it == B.ghostMap.end()) throw GeometryError("image-point box holds a ghost of another body");
+                    const Ghost& o = B.ghosts[it->second];
                     V3 nrm = o.ip - o.bi;
                     nrm = nrm / length(nrm);
                     normals[nNormals++] = nrm;
@@ -758,7 +799,7 @@ public:
     // Mesh::applyAllBoundaryConditions, src/Mesh.cpp:155-167
     void applyAllBCs(double tnow) {
         for (const Edge& e : edges) applyEdge(e, tnow);
-        if (P.body_type != NS_BODY_NONE) applyImmersed();
+        for (const Body& B : bodies) applyImmersed(B);
     }
 
     // ---------------------------------------------------------------- filter, src/Mesh.cpp:85-140
@@ -824,6 +865,7 @@ public:
 
     void initialize() {  // Solver::initialize, src/Solver.cpp:22-28
         setupBoundaries();
+        setupBodies();
         categorizeNodes();
         applyIC();
         updateTimeStepSize(0);

@@ -177,6 +177,17 @@ int main(int argc, char** argv) {
         CHECK(ns_oracle_initialize(h) == NS_ERR_GEOMETRY);
         ns_oracle_destroy(h);
     }
+    // 4b. three bodies (two spheres and a z-cylinder), each with its own ghost map and filter list
+    {
+        ns_config c = sideWalls(48, 40, 26, 4.8);
+        c.L_y = 4.0; c.L_z = 2.6;
+        c.body_centre[0] = 1.0; c.body_centre[1] = 1.0; c.body_centre[2] = 1.3; c.body_radius = 0.35;
+        c.n_extra_bodies = 2;
+        c.extra_bodies[0] = ns_body{NS_BODY_SPHERE, 0, {1.0, 3.0, 1.3}, 0.35};
+        c.extra_bodies[1] = ns_body{NS_BODY_CYLINDER, 2, {3.4, 2.0, 0.0}, 0.4};
+        nsfsi::setDerived(c);
+        exercise("three bodies", c, 3);
+    }
     // 5. fully periodic box, filter every 3 steps
     {
         ns_config c = sideWalls(14, 15, 16, 1.0);

@@ -25,7 +25,7 @@ def test_every_declared_symbol_is_exported():
     assert len(syms) >= 20
     for s in syms:
         assert hasattr(L, s), s
-    assert L.ns_abi_version() == 1
+    assert L.ns_abi_version() == 2
 
 
 def test_struct_layout_matches_header():

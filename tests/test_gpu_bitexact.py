@@ -46,7 +46,7 @@ def assert_bit_identical(s, o, steps):
 
 @pytest.mark.parametrize("name,steps", [("sphere24", 20), ("sphere48", 20), ("cylinder_z64", 10),
                                         ("sphere_offcentre33", 15), ("sphere_seam100", 15), ("sphere_inflow_y", 15),
-                                        ("cylinder_x_inflow_z", 15)])
+                                        ("cylinder_x_inflow_z", 15), ("two_bodies", 15), ("three_bodies", 10)])
 def test_bit_identical_trajectory(name, steps):
     cfg = CLASSIFY_CASES[name]()
     s, o = run_pair(cfg, steps)

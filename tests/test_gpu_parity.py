@@ -87,6 +87,10 @@ CLASSIFY_CASES = {
         body_type="cylinder", body_axis="x", body_centre=(1.0, 0.98, 1.31), body_radius=0.4, stop_criterion="timesteps",
         stop_time_level=5, filter_intervals=[0], filter_change_times=[], ic_type="freestream_gaussian",
         ic_amplitude=1e-3, ic_seed=17, M_0=0.25, Re=200.0, conv_stability_limit=0.5),
+    # more than one immersed body (src/Mesh.cpp:52-57): each with its own ghost map and filter list,
+    # classified and applied in list order
+    "two_bodies": lambda: cases.two_bodies(steps=15),
+    "three_bodies": lambda: cases.three_bodies(steps=10),
 }
 
 
@@ -105,14 +109,15 @@ def test_classification_bit_exact(name):
     assert np.array_equal(bi, obi[order])  # bit-exact FP64 geometry
     assert np.array_equal(ip, oip[order])
     assert np.array_equal(box, obox[order])
-    assert np.array_equal(s.mesh.filterNodes(), o.filter_nodes())
+    ofn = o.filter_nodes()  # the oracle lists them body after body, the device by reference index
+    assert np.array_equal(s.mesh.filterNodes(), ofn[np.argsort(key(ofn), kind="stable")])
     sp, of = o.geometry()
     assert s.mesh.dx == sp[0] and s.mesh.dy == sp[1] and s.mesh.dz == sp[2]
     assert np.array_equal(s.mesh.positionOffset, of)
 
 
 @pytest.mark.parametrize("name", ["sphere24", "sphere48", "cylinder_z64", "sphere_offcentre33", "sphere_seam100",
-                                  "sphere_inflow_y", "cylinder_x_inflow_z"])
+                                  "sphere_inflow_y", "cylinder_x_inflow_z", "two_bodies", "three_bodies"])
 def test_initial_state_bit_exact(name):
     cfg = CLASSIFY_CASES[name]()
     s, o = _pair(cfg)
@@ -123,7 +128,7 @@ def test_initial_state_bit_exact(name):
 @pytest.mark.parametrize("kernel", [1, 0], ids=["fused", "unfused"])
 @pytest.mark.parametrize("name,steps", [("sphere24", 20), ("sphere48", 20), ("cylinder_z64", 10),
                                         ("sphere_offcentre33", 15), ("sphere_seam100", 15), ("sphere_inflow_y", 15),
-                                        ("cylinder_x_inflow_z", 15)])
+                                        ("cylinder_x_inflow_z", 15), ("two_bodies", 15), ("three_bodies", 10)])
 def test_trajectory_fp64(name, steps, kernel):
     cfg = CLASSIFY_CASES[name]()
     s, o = _pair(cfg)
@@ -179,6 +184,85 @@ def test_geometry_error_matches_oracle():
     with pytest.raises(nsfsi.NSError) as e:
         s.initialize()
     assert e.value.status == 2
+
+
+@pytest.mark.parametrize("precision,kernel", [("fp64", 1), ("fp32", 1), ("fp32", 0)],
+                         ids=["fp64-fused", "fp32-fused", "fp32-unfused"])
+def test_symmetry_faces_through_a_body_every_step(precision, kernel):
+    """A z-cylinder crossing both symmetry z-faces: the face nodes inside the body mirror the stored primitive
+    values of untouched solid nodes two in, which stay at their initial values while the conserved values
+    there alternate with the zero-initialised array. The forms that derive primitives from the state write
+    those face nodes from constant tables; after every one of the first 4 steps they equal the oracle's."""
+    def make(prec):
+        # z spacing 1/8: the image points of ghosts on the z-faces sit exactly on those faces
+        return nsfsi.make_config(
+            NI=40, NJ=30, NK=14, L_x=4.0, L_y=3.0, L_z=1.375, bc=cases.SIDE_WALLS, body_type="cylinder",
+            body_axis="z", body_centre=(1.6, 1.4, 0.0), body_radius=0.45, stop_criterion="timesteps",
+            stop_time_level=4, filter_intervals=[0], filter_change_times=[], ic_type="freestream_gaussian",
+            ic_amplitude=1e-3, ic_seed=31, M_0=0.3, Re=300.0, conv_stability_limit=0.5, precision=prec)
+
+    cfg, ref = make(precision), make("fp64")
+    o = oracle.Oracle(ref)
+    o.initialize()
+    nt = o.node_types()
+    face_solid = np.zeros_like(nt, dtype=bool)
+    face_solid[[0, -1]] = nt[[0, -1]] == 3
+    assert face_solid.sum() > 0
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    s.setOption("stage_kernel", kernel)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    for step in range(1, 5):
+        s.marchTimeStep(1)
+        o.march(1)
+        U, V = s.mesh.conserved(), o.conserved()
+        d = np.abs(U - V)[:, face_solid].max()
+        scale = np.abs(V[:, face_solid]).max()
+        print(f"step {step}: face solid nodes max |diff| {d:.3e} (scale {scale:.3e})", flush=True)
+        assert d <= tol * max(scale, 1e-300), step
+        assert max(rel_l2(U[q], V[q]) for q in range(5)) <= (1e-11 if precision == "fp64" else 2e-6), step
+
+
+def test_overlapping_body_boxes_refused():
+    """The device classifies all bodies in one pass, which equals the reference's body-by-body classification
+    only when the bodies' bounding boxes are disjoint: overlapping boxes are refused at initialisation."""
+    cfg = cases.two_bodies()
+    cfg.extra_bodies[0].centre[0] = 2.3  # the cylinder's box now overlaps the sphere's
+    s = nsfsi.Solver(cfg)
+    with pytest.raises(nsfsi.NSError) as e:
+        s.initialize()
+    assert e.value.status == 2 and "overlap" in str(e.value)
+
+
+@pytest.mark.parametrize("kernel", [1, 0], ids=["fused", "unfused"])
+def test_two_bodies_fp32_compensated(kernel):
+    """FP32 with the compensated combine on the two-body case, against the FP64 oracle after 15 steps, to the
+    FP32 bar of the 48^3 sphere (2e-6 relative L2 per conserved field)."""
+    cfg = cases.two_bodies(steps=15, precision="fp32")
+    cfg.kahan = 1
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    s.setOption("stage_kernel", kernel)
+    s.marchTimeStep(15)
+    o = oracle.Oracle(cases.two_bodies(steps=15))
+    o.initialize()
+    o.march(15)
+    U, V = s.mesh.conserved(), o.conserved()
+    errs = [rel_l2(U[q], V[q]) for q in range(5)]
+    print("FP32 two bodies rel L2", errs)
+    assert max(errs) <= 2e-6
+
+
+def test_multi_body_refused_by_batch_and_adjoint():
+    cfg = cases.two_bodies(precision="fp32")
+    with pytest.raises(nsfsi.NSError) as e:
+        nsfsi.Batch([cfg, cfg])
+    assert "one immersed body" in str(e.value)
+    s = nsfsi.Solver(cases.two_bodies())
+    s.initialize()
+    with pytest.raises(nsfsi.NSError) as e:
+        s.adjointDrag(2, 0, 2)
+    assert "one immersed body" in str(e.value)
 
 
 def test_fused_matches_unfused_fp32():

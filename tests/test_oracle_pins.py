@@ -378,67 +378,73 @@ def brute_force_classes(cfg):
     X = (ii - off[0]) * h[0]
     Y = (jj - off[1]) * h[1]
     Z = (kk - off[2]) * h[2]
-    c = list(cfg.body_centre)
-    r = cfg.body_radius
     mpb = np.finfo(float).eps * (cfg.L_x + cfg.L_y + cfg.L_z)
-    if cfg.body_type == 2:
-        d = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2)
-        fw = 2 * max(h)
-        axes = [0, 1, 2]
-    else:
-        ax = cfg.body_axis
-        comps = [(X - c[0]) ** 2, (Y - c[1]) ** 2, (Z - c[2]) ** 2]
-        d = np.sqrt(sum(comps[q] for q in range(3) if q != ax))
-        fw = 2 * max(h[q] for q in range(3) if q != ax)
-        axes = [q for q in range(3) if q != ax]
-    inbox = np.ones_like(t, dtype=bool)
-    for a in axes:
-        rad = int(np.ceil(r / h[a])) + 3
-        ci = int(np.round(c[a] / h[a]))
-        idx = [ii, jj, kk][a]
-        inbox &= (idx >= ci - rad) & (idx <= ci + rad)
-    solid = inbox & (d < r - mpb)
-    filt = inbox & ~solid & (d < r + fw) & (t == 0)
-    t[solid] = 3
-    # ghost rule: 18 neighbours
-    fluid = np.pad(t == 0, 1)
-    has = np.zeros_like(solid)
-    for di in (-1, 0, 1):
-        for dj in (-1, 0, 1):
-            for dk in (-1, 0, 1):
-                nz = (di != 0) + (dj != 0) + (dk != 0)
-                if nz in (0, 3):
-                    continue
-                has |= fluid[1 + di:1 + di + NI, 1 + dj:1 + dj + NJ, 1 + dk:1 + dk + NK]
-    t[solid & has] = 4
-    # iterative augmentation: every SolidInactive node of an image-point box becomes a ghost
-    cvec = np.array(c)
-    while True:
-        g = np.argwhere(t == 4)
-        pos = (g - np.array(off)) * np.array(h)
-        c2g = pos - cvec
-        if cfg.body_type == 1:
-            c2g[:, cfg.body_axis] = 0
-        ln = np.sqrt((c2g ** 2).sum(axis=1))
-        probe = c2g * ((r - ln) / ln)[:, None]
-        ip = pos + 2 * probe
-        nxt = np.ceil(ip / np.array(h)) + np.array(off)
-        nxt = np.clip(nxt, 1, np.array(Ns)).astype(int)
-        new = 0
-        for q in range(len(g)):
-            for a in (0, 1):
-                for b in (0, 1):
-                    for e in (0, 1):
-                        n = (nxt[q, 0] - 1 + a, nxt[q, 1] - 1 + b, nxt[q, 2] - 1 + e)
-                        if t[n] == 3:
-                            t[n] = 4
-                            new += 1
-        if new == 0:
-            break
+    bodies = [(cfg.body_type, cfg.body_axis, list(cfg.body_centre), cfg.body_radius)]
+    for q in range(cfg.n_extra_bodies):
+        e = cfg.extra_bodies[q]
+        bodies.append((e.type, e.axis, list(e.centre), e.radius))
+    filt = np.zeros_like(t, dtype=bool)
+    for btype, bax, c, r in bodies:  # one body after the other (src/Mesh.cpp:73-74)
+        if btype == 2:
+            d = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2)
+            fw = 2 * max(h)
+            axes = [0, 1, 2]
+        else:
+            comps = [(X - c[0]) ** 2, (Y - c[1]) ** 2, (Z - c[2]) ** 2]
+            d = np.sqrt(sum(comps[q] for q in range(3) if q != bax))
+            fw = 2 * max(h[q] for q in range(3) if q != bax)
+            axes = [q for q in range(3) if q != bax]
+        inbox = np.ones_like(t, dtype=bool)
+        for a in axes:
+            rad = int(np.ceil(r / h[a])) + 3
+            ci = int(np.round(c[a] / h[a]))
+            idx = [ii, jj, kk][a]
+            inbox &= (idx >= ci - rad) & (idx <= ci + rad)
+        solid = inbox & (d < r - mpb)
+        filt |= inbox & ~solid & (d < r + fw) & (t == 0)
+        t[solid] = 3
+        # ghost rule: 18 neighbours
+        fluid = np.pad(t == 0, 1)
+        has = np.zeros_like(solid)
+        for di in (-1, 0, 1):
+            for dj in (-1, 0, 1):
+                for dk in (-1, 0, 1):
+                    nz = (di != 0) + (dj != 0) + (dk != 0)
+                    if nz in (0, 3):
+                        continue
+                    has |= fluid[1 + di:1 + di + NI, 1 + dj:1 + dj + NJ, 1 + dk:1 + dk + NK]
+        mine = solid & has  # this body's ghosts
+        t[mine] = 4
+        # iterative augmentation: every SolidInactive node of an image-point box of this body's ghosts becomes
+        # one of its ghosts
+        cvec = np.array(c)
+        while True:
+            g = np.argwhere(mine)
+            pos = (g - np.array(off)) * np.array(h)
+            c2g = pos - cvec
+            if btype == 1:
+                c2g[:, bax] = 0
+            ln = np.sqrt((c2g ** 2).sum(axis=1))
+            probe = c2g * ((r - ln) / ln)[:, None]
+            ip = pos + 2 * probe
+            nxt = np.ceil(ip / np.array(h)) + np.array(off)
+            nxt = np.clip(nxt, 1, np.array(Ns)).astype(int)
+            new = 0
+            for q in range(len(g)):
+                for a in (0, 1):
+                    for b in (0, 1):
+                        for e in (0, 1):
+                            n = (nxt[q, 0] - 1 + a, nxt[q, 1] - 1 + b, nxt[q, 2] - 1 + e)
+                            if t[n] == 3:
+                                t[n] = 4
+                                mine[n] = True
+                                new += 1
+            if new == 0:
+                break
     return t, filt
 
 
-@pytest.mark.parametrize("name", ["sphere", "cylinder_z", "cylinder_x", "offcentre"])
+@pytest.mark.parametrize("name", ["sphere", "cylinder_z", "cylinder_x", "offcentre", "two_bodies", "three_bodies"])
 def test_classification_brute_force(name):
     if name == "sphere":
         cfg = cases.sphere48(N=24, steps=1)
@@ -448,6 +454,10 @@ def test_classification_brute_force(name):
         cfg = nsfsi.make_config(NI=30, NJ=26, NK=28, L_x=3.0, L_y=2.0, L_z=2.2, bc=cases.SIDE_WALLS,
                                 body_type="cylinder", body_axis="x", body_centre=(0.0, 1.03, 1.11), body_radius=0.41,
                                 filter_intervals=[0], filter_change_times=[])
+    elif name == "two_bodies":
+        cfg = cases.two_bodies()
+    elif name == "three_bodies":
+        cfg = cases.three_bodies()
     else:
         cfg = nsfsi.make_config(NI=33, NJ=29, NK=31, L_x=3.0, L_y=2.5, L_z=2.7, bc=cases.SIDE_WALLS, body_type="sphere",
                                 body_centre=(1.31, 1.27, 1.33), body_radius=0.61, filter_intervals=[0],
@@ -462,6 +472,71 @@ def test_classification_brute_force(name):
     assert np.array_equal(got, filt)
     g = o.counts()[0]
     assert g == int((t == 4).sum())
+
+
+def test_two_bodies_overlay_single_body_runs():
+    """With disjoint boxes, the two-body classes are the overlay of each body classified alone, the ghost and
+    filter lists are the single-body lists one after the other, in body order."""
+    cfg = cases.two_bodies()
+    o = oracle.Oracle(cfg)
+    o.initialize()
+    first = nsfsi.make_config(**{k: getattr(cfg, k) for k, _ in nsfsi.NSConfig._fields_
+                                 if k not in ("extra_bodies", "n_extra_bodies", "pad_bodies")})
+    e = cfg.extra_bodies[0]
+    second = nsfsi.make_config(**{k: getattr(cfg, k) for k, _ in nsfsi.NSConfig._fields_
+                                  if k not in ("extra_bodies", "n_extra_bodies", "pad_bodies", "body_type",
+                                               "body_axis", "body_centre", "body_radius")})
+    second.body_type, second.body_axis, second.body_radius = e.type, e.axis, e.radius
+    for a in range(3):
+        second.body_centre[a] = e.centre[a]
+    runs = []
+    for c in (first, second):
+        r = oracle.Oracle(c)
+        r.initialize()
+        runs.append(r)
+    t0, t1 = runs[0].node_types(), runs[1].node_types()
+    body0, body1 = t0 >= 3, t1 >= 3
+    assert not np.any(body0 & body1)
+    expect = np.where(body0, t0, t1)
+    assert np.array_equal(o.node_types(), expect)
+    ijk, bi, ip, box = o.ghosts()
+    n0 = runs[0].counts()[0]
+    for mine, part in zip((ijk[:n0], bi[:n0], ip[:n0], box[:n0]), runs[0].ghosts()):
+        assert np.array_equal(mine, part)
+    for mine, part in zip((ijk[n0:], bi[n0:], ip[n0:], box[n0:]), runs[1].ghosts()):
+        assert np.array_equal(mine, part)
+    assert np.array_equal(o.filter_nodes(), np.concatenate([runs[0].filter_nodes(), runs[1].filter_nodes()]))
+
+
+def test_mirror_symmetry_of_two_mirrored_spheres():
+    """Two equal spheres mirrored about the y midplane (all coordinates multiples of 1/16, so the mirrored
+    distances are exact): classes and the state after 8 steps are mirror images to rounding, v odd. Each body's ghosts
+    and filter nodes must be applied with its own geometry for this to hold."""
+    cfg = nsfsi.make_config(NI=33, NJ=35, NK=19, L_x=2.0, L_y=2.0, L_z=1.0, bc=cases.SIDE_WALLS, body_type="sphere",
+                            body_centre=(0.75, 0.5, 0.5), body_radius=0.22,
+                            extra_bodies=[dict(type="sphere", centre=(0.75, 1.5, 0.5), radius=0.22)],
+                            ic_type="freestream_uniform", M_0=0.3, Re=300.0, conv_stability_limit=0.5,
+                            stop_criterion="timesteps", stop_time_level=8, filter_intervals=[0],
+                            filter_change_times=[])
+    o = oracle.Oracle(cfg)
+    o.initialize()
+    nt = o.node_types()
+    assert (nt == 4).sum() > 0 and np.array_equal(nt, nt[:, ::-1, :])
+    o.march(8)
+    rho, ru, rv, rw, rE = o.conserved()
+    sc = np.abs(ru).max()
+    assert sc > 1e-3
+    # rounding of the mirrored stencil sums (the j+1 and j-1 terms swap places) stays near 1e-12 of each field
+    for f in (rho, ru, rw, rE):
+        assert np.allclose(f, f[:, ::-1, :], atol=1e-11 * np.abs(f).max())
+    assert np.allclose(rv, -rv[:, ::-1, :], atol=1e-11 * sc)
+    # the second sphere matters: the one-sphere run differs near it
+    one = nsfsi.make_config(**{k: getattr(cfg, k) for k, _ in nsfsi.NSConfig._fields_
+                               if k not in ("extra_bodies", "n_extra_bodies", "pad_bodies")})
+    o1 = oracle.Oracle(one)
+    o1.initialize()
+    o1.march(8)
+    assert np.abs(o1.field(0) - rho).max() > 1e-4
 
 
 def test_sphere_solid_volume():
