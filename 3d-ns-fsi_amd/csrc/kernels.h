@@ -43,6 +43,8 @@ struct GhostTables {
     double* luD;        // 64 per ghost: LU factors of the Dirichlet Vandermonde matrix
     double* luN;        // 64 per ghost: LU factors of the Neumann Vandermonde matrix
     int8_t* perm;       // 32 per ghost: row/col permutations (D rows, D cols, N rows, N cols)
+    double* wD;         // 8 per ghost: image-point weights of the Dirichlet interpolant, b(IP)^T V_D^-1
+    double* wN;         // 8 per ghost: the same for the Neumann interpolant
     int32_t* boxLo;     // 3 per ghost: lower corner (i,j,k)
 };
 

@@ -232,6 +232,25 @@ __device__ void lu8Factor(double* A, int8_t* rp, int8_t* cp) {
     }
 }
 
+// Weights w with w . f = b . c for the solution c of V c = f (luSolveDot below): w = P^T L^-T U^-T Q^T b,
+// the transposed substitutions in reverse order, with the same treatment of a zero pivot (that component of
+// the solution is 0). The interpolated value is then one dot product with the 8 corner values.
+__device__ void luWeights(const double* LU, const int8_t* rp, const int8_t* cp, const double* b, double* w) {
+    double t[8], s[8], v[8];
+    for (int r = 0; r < 8; ++r) t[r] = b[cp[r]];
+    for (int r = 0; r < 8; ++r) {  // U^T s = t (forward)
+        double acc = t[r];
+        for (int q = 0; q < r; ++q) acc -= LU[q * 8 + r] * s[q];
+        s[r] = LU[r * 8 + r] != 0 ? acc / LU[r * 8 + r] : 0.0;
+    }
+    for (int r = 7; r >= 0; --r) {  // L^T v = s (backward, unit diagonal)
+        double acc = s[r];
+        for (int q = r + 1; q < 8; ++q) acc -= LU[q * 8 + r] * v[q];
+        v[r] = acc;
+    }
+    for (int r = 0; r < 8; ++r) w[rp[r]] = v[r];
+}
+
 // Image-point geometry, corner tables and both Vandermonde factorisations for every ghost
 // (setInterpolationValues + populateVandermonde*, src/Boundary.cpp:655-728, :788-817).
 __global__ void k_ghost_geometry(Layout L, const uint8_t* cls, Params P, Window W, const int64_t* ghosts, int64_t base,
@@ -294,6 +313,9 @@ __global__ void k_ghost_geometry(Layout L, const uint8_t* cls, Params P, Window 
     int8_t* pm = G.perm + 32 * g;
     lu8Factor(D, pm, pm + 8);
     lu8Factor(N, pm + 16, pm + 24);
+    const double b[8] = {1, qx, qy, qz, qx * qy, qx * qz, qy * qz, qx * qy * qz};
+    luWeights(D, pm, pm + 8, b, G.wD + 8 * g);
+    luWeights(N, pm + 16, pm + 24, b, G.wN + 8 * g);
 }
 
 // ------------------------------------------------------------------------------------------------ BCs
@@ -602,6 +624,12 @@ __device__ __forceinline__ void ghostApplyField(const Layout& L, const Params& P
         const double a0123 = a01 + (a23 - a01) * ry / P.hy;
         const double a4567 = a45 + (a67 - a45) * ry / P.hy;
         ipv = a0123 + (a4567 - a0123) * rx / P.hx;
+    } else if constexpr (sizeof(T) == 4) {
+        // FP32 fields: the precomputed image-point weights (one dot product; the same linear map as the
+        // substitutions, rounded differently)
+        const double* w = (q == 0 || q == 4 ? G.wN : G.wD) + 8 * g;
+        ipv = 0;
+        for (int c = 0; c < 8; ++c) ipv += w[c] * f[c];
     } else {
         // density and energy: Neumann factors; momentum: Dirichlet
         const bool neu = q == 0 || q == 4;
@@ -712,6 +740,8 @@ GhostTables ghostTablesFrom(const GhostTables& G, int64_t b) {
     o.luN = G.luN + 64 * b;
     o.perm = G.perm + 32 * b;
     o.boxLo = G.boxLo + 3 * b;
+    o.wD = G.wD + 8 * b;
+    o.wN = G.wN + 8 * b;
     return o;
 }
 

@@ -343,6 +343,8 @@ void Case::categorizeNodes() {
     G.luD = (double*)alloc(64 * nGhost * sizeof(double));
     G.luN = (double*)alloc(64 * nGhost * sizeof(double));
     G.perm = (int8_t*)alloc(32 * nGhost);
+    G.wD = (double*)alloc(8 * nGhost * sizeof(double));
+    G.wN = (double*)alloc(8 * nGhost * sizeof(double));
     G.boxLo = (int32_t*)alloc(3 * nGhost * sizeof(int32_t));
     hipCheck(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), stream), "memset flags");
     for (int b = 0; b < bodies.n; ++b)
