@@ -1108,7 +1108,7 @@ void Case::getField(int which, double* out) {
 namespace {
 
 constexpr uint64_t kCkptMagic = 0x31544b5049534e4eULL;  // "NNSIPKT1"
-constexpr uint32_t kCkptVersion = 2;
+constexpr uint32_t kCkptVersion = 3;
 
 // Configuration fields a checkpoint must agree on: the grid, boundaries, body, initial-state generator
 // and physics (stopping, output and stability-limit settings may change between runs).
@@ -1140,7 +1140,7 @@ struct CkptHeader {
     uint64_t magic;
     uint32_t version, primsStored;
     CkptKey key;
-    uint64_t fstride, windowSize, nInactive, nNorms, nForces;
+    uint64_t fstride, windowSize, nInactive, nNorms, nForces, nFixed;
     int32_t conv_history, pad;
 };
 
@@ -1159,6 +1159,7 @@ void Case::save(const std::string& path) {
     hd.fstride = L.fstride;
     hd.windowSize = W.size();
     hd.nInactive = (uint64_t)nInactive;
+    hd.nFixed = (uint64_t)nFixed;
     hd.nNorms = normsHost[0].size();
     hd.nForces = forceHost.size() / 4;
     hd.conv_history = cfg.conv_history;
@@ -1183,6 +1184,7 @@ void Case::save(const std::string& path) {
     putDevice(rhoEPre, wb);
     if (comp) putDevice(comp, 5 * L.fstride * sizeof(int8_t));
     putDevice(inactiveIC, 5 * std::max<int64_t>(1, nInactive) * sizeof(double));
+    if (nFixed) putDevice(fixedVals, 5 * nFixed * sizeof(double));
     StepScalars st;
     hipCheck(hipMemcpy(&st, scal, sizeof(st), hipMemcpyDeviceToHost), "checkpoint scalars");
     put(&st, sizeof(st));
@@ -1205,7 +1207,7 @@ void Case::restore(const std::string& path) {
     if (std::memcmp(&hd.key, &want, sizeof(want)) != 0)
         throw std::invalid_argument("checkpoint was written by a case with another grid, boundaries, body or physics");
     if (hd.fstride != L.fstride || hd.windowSize != W.size() || hd.nInactive != (uint64_t)nInactive ||
-        hd.conv_history != cfg.conv_history)
+        hd.nFixed != (uint64_t)nFixed || hd.conv_history != cfg.conv_history)
         throw std::invalid_argument("checkpoint layout does not match this case");
     dropGraphs();
     std::vector<char> host;
@@ -1222,6 +1224,7 @@ void Case::restore(const std::string& path) {
     getDevice(rhoEPre, wb);
     if (comp) getDevice(comp, 5 * L.fstride * sizeof(int8_t));
     getDevice(inactiveIC, 5 * std::max<int64_t>(1, nInactive) * sizeof(double));
+    if (nFixed) getDevice(fixedVals, 5 * nFixed * sizeof(double));
     StepScalars st;
     get(&st, sizeof(st));
     // the device rings restart empty; the histories so far are kept on the host

@@ -68,6 +68,37 @@ def test_restart_is_bitwise_continuation(tmp_path, form, graph):
     c.close()
 
 
+@pytest.mark.parametrize("form", ["fp64_fused", "fp32_compensated_fused"])
+def test_restart_with_two_bodies_and_a_body_through_the_symmetry_faces(tmp_path, form):
+    """The two-body case (its z-cylinder crosses both symmetry z-faces, whose nodes inside the body hold the
+    constant mirrored values of untouched solid nodes): restored from a checkpoint written by a run with
+    another initial field seed, the run still continues bit for bit (the checkpoint carries those values)."""
+    prec, kernel, kahan = FORMS[form]
+
+    def cfg_of(seed, steps=12):
+        c = cases.two_bodies(steps=steps, precision=prec)
+        c.kahan = kahan
+        c.ic_seed = seed
+        return c
+
+    print(f"checkpoint two bodies {form}", flush=True)
+    a = solver(cfg_of(23), kernel, 1)
+    a.marchTimeStep(9)
+    b = solver(cfg_of(23), kernel, 1)
+    b.marchTimeStep(4)
+    path = tmp_path / "two.ckpt"
+    b.save(path)
+    b.close()
+    c = solver(cfg_of(99, steps=30), kernel, 1)
+    c.restore(path)
+    c.marchTimeStep(5)
+    assert a.time() == c.time()
+    for name in nsfsi.FIELDS[:5]:
+        assert np.array_equal(a.mesh.field(name), c.mesh.field(name)), name
+    a.close()
+    c.close()
+
+
 def test_checkpoint_refused_by_another_case(tmp_path):
     cfg, kernel = make("fp64_fused")
     s = solver(cfg, kernel, 0)
