@@ -58,6 +58,7 @@ void* Batch::alloc(size_t bytes) {
 }
 
 void Batch::dropGraphs() {
+    if (graphExec[0] || graphExec[1]) hipCheck(hipStreamSynchronize(stream), "sync before dropping graphs");
     for (int q = 0; q < 2; ++q) {
         if (graphExec[q]) (void)hipGraphExecDestroy(graphExec[q]);
         graphExec[q] = nullptr;

@@ -988,6 +988,7 @@ void Case::readScalars() {
 }
 
 void Case::dropGraphs() {
+    if (graphExec[0] || graphExec[1]) hipCheck(hipStreamSynchronize(stream), "sync before dropping graphs");
     for (int q = 0; q < 2; ++q) {
         if (graphExec[q]) (void)hipGraphExecDestroy(graphExec[q]);
         graphExec[q] = nullptr;
