@@ -282,10 +282,13 @@ double Batch::autotune(int reps) {
             tile.ty = tys[a];
             tile.kc = kcs[b];
             buildStageTables();
-            // stage 2 writes S2 and the accumulator's interior, which stage 1 of the next step rewrites
+            // stages 1-3 write S1, S2 and the accumulator's interior, which the next step rewrites in the same
+            // order before reading them (U, the accumulator's other nodes and the carry are untouched)
             auto run = [&](int count) {
                 for (int r = 0; r < count; ++r)
-                    launchStageBatched(stream, 2, tile.ty, false, kahan, devArgs[parity][1], devStageStart, n, totalTiles);
+                    for (int st = 1; st <= 3; ++st)
+                        launchStageBatched(stream, st, tile.ty, false, kahan, devArgs[parity][st - 1], devStageStart, n,
+                                           totalTiles);
             };
             run(1);
             hipCheck(hipEventRecord(e0, stream), "record");

@@ -510,12 +510,13 @@ __global__ void k_edge_faces_batched(const CaseView<T>* __restrict__ cv, EdgeFac
     const int c = findCase(start, ncases, q);
     const CaseView<T>& v = cv[c];
     const EdgeFace& f = v.faces[fl.face[m]];
+    const int64_t r = q - start[c];
+    const double ts = v.scal->t + frac * v.scal->dt;
+    T* S = roleBuffer(v, role, parity);
     const int ax1 = f.axis == 0 ? 1 : 0;
     const int lo1 = ax1 == 0 ? f.i0 : f.j0, hi1 = ax1 == 0 ? f.i1 : f.j1;
-    const int64_t r = q - start[c];
     const int n1 = hi1 - lo1 + 1;
-    const double ts = v.scal->t + frac * v.scal->dt;
-    edgeFaceNode(v.L, v.P, f, roleBuffer(v, role, parity), ts, (int)(r % n1), (int)(r / n1));
+    edgeFaceNode(v.L, v.P, f, S, ts, (int)(r % n1), (int)(r / n1));
 }
 
 // filterClosestFluidNodes, src/Boundary.cpp:618-652, first phase: filtered rho and rho*E into tmp, and
@@ -640,6 +641,49 @@ __device__ __forceinline__ void ghostApplyField(const Layout& L, const Params& P
     Sq[(size_t)G.node[g]] = (T)(q >= 1 && q <= 3 ? -ipv : ipv);
 }
 
+// All five fields of one ghost in one thread (FP32 fields: one dot product per field, so the corner offsets
+// and tables are read once per ghost).
+template <class T>
+__device__ __forceinline__ void ghostApplyAll(const Layout& L, const Params& P, const GhostTables& G, int64_t g, T* S) {
+    const uint8_t mask = G.mask[g];
+    size_t o[8];
+    for (int c = 0; c < 8; ++c) o[c] = (size_t)G.corner[8 * g + c];
+    const size_t node = (size_t)G.node[g];
+    double r[3], wD[8], wN[8];
+    if (mask == 0) {
+        for (int d = 0; d < 3; ++d) r[d] = G.rel[3 * g + d];
+    } else {
+        for (int c = 0; c < 8; ++c) {
+            wD[c] = G.wD[8 * g + c];
+            wN[c] = G.wN[8 * g + c];
+        }
+    }
+    double out[5];
+    for (int q = 0; q < 5; ++q) {
+        const T* Sq = S + (size_t)q * L.fstride;
+        double f[8];
+        for (int c = 0; c < 8; ++c) f[c] = (mask >> c) & 1 ? 0.0 : (double)Sq[o[c]];
+        double ipv;
+        if (mask == 0) {  // simplifiedInterpolation, src/Boundary.cpp:732-757
+            const double a01 = f[0] + (f[1] - f[0]) * r[2] / P.hz;
+            const double a23 = f[2] + (f[3] - f[2]) * r[2] / P.hz;
+            const double a45 = f[4] + (f[5] - f[4]) * r[2] / P.hz;
+            const double a67 = f[6] + (f[7] - f[6]) * r[2] / P.hz;
+            const double a0123 = a01 + (a23 - a01) * r[1] / P.hy;
+            const double a4567 = a45 + (a67 - a45) * r[1] / P.hy;
+            ipv = a0123 + (a4567 - a0123) * r[0] / P.hx;
+        } else {
+            const double* w = q == 0 || q == 4 ? wN : wD;
+            ipv = 0;
+            for (int c = 0; c < 8; ++c) ipv += w[c] * f[c];
+        }
+        out[q] = q >= 1 && q <= 3 ? -ipv : ipv;
+    }
+    // a ghost corner enters as its boundary value, never as its stored value, so the writes of other ghosts
+    // of the same launch do not reach this ghost's inputs
+    for (int q = 0; q < 5; ++q) S[(size_t)q * L.fstride + node] = (T)out[q];
+}
+
 // Stored primitive and transport values of the ghosts after their update (src/Boundary.cpp:447-455:
 // derivePrimitiveVariables and deriveTransportProperties of the ghost's conserved values).
 template <class T>
@@ -654,21 +698,30 @@ __global__ void k_ghost_prims(Layout L, Params P, GhostTables G, int64_t n, cons
     storePrims(L, prim, o, u, v, w, p, Tt, mu, mu / e.gm1Pr);
 }
 
-template <class T>
+// Ghosts per launch from which a thread per ghost (all five fields) beats a thread per (ghost, field) in
+// FP32: below it the launch is latency-bound and five times the threads finish sooner.
+constexpr int64_t kGhostPerThread = 65536;
+
+template <class T, bool PER_GHOST>
 __global__ void k_ghost_apply(Layout L, Params P, GhostTables G, int64_t n, T* S) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < 5 * n) ghostApplyField(L, P, G, t / 5, (int)(t % 5), S);
+    if constexpr (PER_GHOST) {
+        if (t < n) ghostApplyAll(L, P, G, t, S);
+    } else {
+        if (t < 5 * n) ghostApplyField(L, P, G, t / 5, (int)(t % 5), S);
+    }
 }
 
-template <class T>
+template <class T, bool PER_GHOST>
 __global__ void k_ghost_apply_batched(const CaseView<T>* __restrict__ cv, const int64_t* __restrict__ start, int ncases,
                                       int role, int parity) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t q = t / 5;
+    const int64_t q = PER_GHOST ? t : t / 5;
     if (q >= start[ncases]) return;
     const int c = findCase(start, ncases, q);
     const CaseView<T>& v = cv[c];
-    ghostApplyField(v.L, v.P, v.G, q - start[c], (int)(t % 5), roleBuffer(v, role, parity));
+    if constexpr (PER_GHOST) ghostApplyAll(v.L, v.P, v.G, q - start[c], roleBuffer(v, role, parity));
+    else ghostApplyField(v.L, v.P, v.G, q - start[c], (int)(t % 5), roleBuffer(v, role, parity));
 }
 
 inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -839,7 +892,10 @@ void launchFilterWrite(hipStream_t s, const Layout& L, const int64_t* fn, int64_
 template <class T>
 void launchGhostApply(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, T* S, T* prim) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_ghost_apply<T>, dim3(blocks(5 * n, 64)), dim3(64), 0, s, L, P, G, n, S);
+    if (sizeof(T) == 4 && n >= kGhostPerThread)
+        hipLaunchKernelGGL((k_ghost_apply<T, true>), dim3(blocks(n, 64)), dim3(64), 0, s, L, P, G, n, S);
+    else
+        hipLaunchKernelGGL((k_ghost_apply<T, false>), dim3(blocks(5 * n, 64)), dim3(64), 0, s, L, P, G, n, S);
     if (prim) hipLaunchKernelGGL(k_ghost_prims<T>, dim3(blocks(n, 64)), dim3(64), 0, s, L, P, G, n, (const T*)S, prim);
 }
 
@@ -871,8 +927,12 @@ template <class T>
 void launchGhostApplyBatched(hipStream_t s, const CaseView<T>* cv, const int64_t* start, int64_t total, int ncases,
                              int role, int parity) {
     if (total == 0) return;
-    hipLaunchKernelGGL(k_ghost_apply_batched<T>, dim3(blocks(5 * total, 64)), dim3(64), 0, s, cv, start, ncases, role,
-                       parity);
+    if (sizeof(T) == 4 && total >= kGhostPerThread)
+        hipLaunchKernelGGL((k_ghost_apply_batched<T, true>), dim3(blocks(total, 64)), dim3(64), 0, s, cv, start, ncases,
+                           role, parity);
+    else
+        hipLaunchKernelGGL((k_ghost_apply_batched<T, false>), dim3(blocks(5 * total, 64)), dim3(64), 0, s, cv, start,
+                           ncases, role, parity);
 }
 
 #define NS_INST(T)                                                                                                           \

@@ -780,8 +780,9 @@ const EdgeFace* Case::stageOutlet() const {
 }
 
 double Case::autotune(int reps) {
-    // stage-2 launches (reads U, A; writes S2 and the interior of A): S2 is rewritten by every step and
-    // A's interior by every stage 1, so the state of the run is unchanged
+    // stages 1-3 of a step (the stage inputs and outputs of the step itself): they write S1, S2 and the
+    // interior of A, which the next step rewrites in the same order before reading them; U, the other nodes
+    // of A and the carry are untouched, so the state of the run is unchanged
     int n = 0;
     const int* tys = fusedTileHeights(&n);
     int nk = 0;
@@ -799,16 +800,20 @@ double Case::autotune(int reps) {
             tl.kc = kcs[b];
             tl.globalAccess = tile.globalAccess;
             auto run = [&](int count) {
-                for (int r = 0; r < count; ++r) {
-                    if (fp32)
-                        launchStageFused<float>(stream, 2, L, P, W, (const float*)U, (const float*)U, (float*)S2, (float*)A,
-                                                cls, (const float*)rhoPre, (const float*)rhoEPre, &scal->dt, tl,
-                                                nullptr, 0, nullptr, nullptr, comp, stageOutlet());
-                    else
-                        launchStageFused<double>(stream, 2, L, P, W, (const double*)U, (const double*)U, (double*)S2,
-                                                 (double*)A, cls, (const double*)rhoPre, (const double*)rhoEPre,
-                                                 &scal->dt, tl, nullptr, 0, nullptr, nullptr, nullptr, stageOutlet());
-                }
+                for (int r = 0; r < count; ++r)
+                    for (int st = 1; st <= 3; ++st) {
+                        void* in = st == 1 ? U : (st == 2 ? S1 : S2);
+                        void* out = st == 2 ? S2 : S1;
+                        if (fp32)
+                            launchStageFused<float>(stream, st, L, P, W, (const float*)in, (const float*)U, (float*)out,
+                                                    (float*)A, cls, (const float*)rhoPre, (const float*)rhoEPre,
+                                                    &scal->dt, tl, nullptr, 0, nullptr, nullptr, comp, stageOutlet());
+                        else
+                            launchStageFused<double>(stream, st, L, P, W, (const double*)in, (const double*)U,
+                                                     (double*)out, (double*)A, cls, (const double*)rhoPre,
+                                                     (const double*)rhoEPre, &scal->dt, tl, nullptr, 0, nullptr, nullptr,
+                                                     nullptr, stageOutlet());
+                    }
             };
             run(1);  // warm-up
             hipCheck(hipEventRecord(e0, stream), "record");
