@@ -145,7 +145,8 @@ __device__ __forceinline__ void stNT(T* base, uint32_t boff, T v) {
 // Field accessors of one stage launch. BUF (FP32 states of at most 4 GiB): buffer instructions, one
 // resource per 5-field state buffer, the node's byte offset in a VGPR and the field's in an SGPR, so no
 // address arithmetic per access. Otherwise (FP64, or an FP32 state beyond 4 GiB): global accesses at
-// per-field base pointers with the node's 32-bit byte offset.
+// per-field base pointers moved to the tile's first plane (kb0, plane kb - 1 or 0), with the node's 32-bit
+// byte offset from there: a tile spans at most its z-chunk plus two planes, so fields of any size work.
 template <class T, bool BUF>
 struct Mem;
 
@@ -160,7 +161,8 @@ template <>
 struct Mem<float, true> {
     __amdgpu_buffer_rsrc_t sin, u, a, sout, comp;
     uint32_t fsb;  // field stride in bytes (FP32)
-    __device__ explicit Mem(const FusedArgs<float>& x) {
+    static constexpr int kb0 = 0;  // offsets from the fields' start
+    __device__ Mem(const FusedArgs<float>& x, int) {
         fsb = (uint32_t)(x.L.fstride * sizeof(float));
         const uint32_t nb = 5 * fsb;
         sin = rsrc(x.Sin[0], nb);
@@ -193,19 +195,33 @@ struct Mem<float, true> {
 
 template <class T>
 struct Mem<T, false> {
-    const FusedArgs<T>& x;
-    __device__ explicit Mem(const FusedArgs<T>& a) : x(a) {}
-    __device__ T ldSin(int q, uint32_t bo) const { return ldAt(x.Sin[q], bo); }
-    __device__ T ldU(int q, uint32_t bo) const { return ldAt(x.U[q], bo); }
-    __device__ T ldA(int q, uint32_t bo) const { return ldAt((const T*)x.A[q], bo); }
-    __device__ void stA(int q, uint32_t bo, T v) const { stNT(x.A[q], bo, v); }
-    __device__ void stSout(int q, uint32_t bo, T v) const { stNT(x.Sout[q], bo, v); }
+    const T* sin[5];
+    const T* u[5];
+    T* a[5];
+    T* sout[5];
+    int8_t* comp[5];
+    int kb0;  // plane the offsets start from
+    __device__ Mem(const FusedArgs<T>& x, int k0) : kb0(k0) {
+        const size_t e = (size_t)k0 * x.L.plane;  // elements (and carry bytes) before plane k0
+        for (int q = 0; q < 5; ++q) {
+            sin[q] = x.Sin[q] + e;
+            u[q] = x.U[q] + e;
+            a[q] = x.A[q] + e;
+            sout[q] = x.Sout[q] ? x.Sout[q] + e : nullptr;
+            comp[q] = x.comp[q] ? x.comp[q] + e : nullptr;
+        }
+    }
+    __device__ T ldSin(int q, uint32_t bo) const { return ldAt(sin[q], bo); }
+    __device__ T ldU(int q, uint32_t bo) const { return ldAt(u[q], bo); }
+    __device__ T ldA(int q, uint32_t bo) const { return ldAt((const T*)a[q], bo); }
+    __device__ void stA(int q, uint32_t bo, T v) const { stNT(a[q], bo, v); }
+    __device__ void stSout(int q, uint32_t bo, T v) const { stNT(sout[q], bo, v); }
     __device__ int8_t ldComp(int q, uint32_t bo) const {
-        if constexpr (sizeof(T) == 4) return ldAt((const int8_t*)x.comp[q], bo >> 2);
+        if constexpr (sizeof(T) == 4) return ldAt((const int8_t*)comp[q], bo >> 2);
         else return 0;
     }
     __device__ void stComp(int q, uint32_t bo, int8_t v) const {
-        if constexpr (sizeof(T) == 4) stNT(x.comp[q], bo >> 2, v);
+        if constexpr (sizeof(T) == 4) stNT(comp[q], bo >> 2, v);
     }
 };
 
@@ -327,7 +343,7 @@ __device__ __forceinline__ Raw<T> fetch(const FusedArgs<T>& a, const M& m, int i
         return r;
     }
     const int kc = min(max(k, 0), L.NK - 1);
-    const uint32_t bo = col + (uint32_t)kc * (uint32_t)(L.plane * sizeof(T));
+    const uint32_t bo = col + (uint32_t)(kc - m.kb0) * (uint32_t)(L.plane * sizeof(T));
     r.rho = m.ldSin(0, bo);
     r.ru = m.ldSin(1, bo);
     r.rv = m.ldSin(2, bo);
@@ -459,7 +475,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     const bool rhoFiltered = STAGE == 1 && a.fFlag && *a.fFlag;
     static_assert(!KAHAN || sizeof(T) == 4, "the compensated combine is the FP32 form");
     static_assert(!BUF || sizeof(T) == 4, "buffer-resource access is the FP32 form");
-    const Mem<T, BUF> m(a);
+    const Mem<T, BUF> m(a, kb > 0 ? kb - 1 : 0);
     // clamped column of the stencil point (halo lanes and rows beyond the grid repeat the edge); the x-tile
     // is segment bx of the layout, so the halo lanes read the neighbouring segments
     const int ic = min(max(i, 0), L.NI - 1), jc = min(max(j, 0), L.NJ - 1);
@@ -528,8 +544,8 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
         }
         // combine inputs of the output node, fetched before the stencil work
         const bool doOut = outRow && outLane;
-        const uint32_t bo = outRow ? colo + (uint32_t)k * planeB : 0;
-        const size_t o = bo / sizeof(T);
+        const uint32_t bo = outRow ? colo + (uint32_t)(k - m.kb0) * planeB : 0;
+        const size_t o = bo / sizeof(T) + (size_t)m.kb0 * L.plane;  // the node's layout offset
         bool interior = false, filterNode = false;
         T base[5], acc[5];
         (void)filterNode;
@@ -623,7 +639,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                         // the reference filters the density of U before the step but keeps the
                         // primitives, so only the combine base changes (src/Solver.cpp:134-141)
                         const T c0 = cur[0];
-                        cur[0] = rhoFiltered ? ldAt(a.fRho, bo) : c0;
+                        cur[0] = rhoFiltered ? ldAt(a.fRho + (size_t)m.kb0 * L.plane, bo) : c0;
                         for (int q = 0; q < 5; ++q) {
                             vS[q] = cur[q] + h2 * rhs[q];
                             vA[q] = KAHAN ? h6 * rhs[q] : cur[q] + h6 * rhs[q];
@@ -960,6 +976,8 @@ void launchBatchedTYK(hipStream_t s, int stage, bool scan, const FusedArgs<T>* t
 int normTY(int ty) { return ty == 4 || ty == 6 || ty == 14 ? ty : 8; }
 
 // Every argument of one stage launch on one case, tile geometry included.
+inline int chunkPlanes(const FusedTile& t) { return t.kc > 0 ? t.kc : 64; }
+
 template <class T>
 void fillArgs(FusedArgs<T>& a, int stage, const Layout& L, const Params& P, const Window& W, const T* Sin, const T* U,
               T* Sout, T* A, const uint8_t* cls, const T* rhoPre, const T* rhoEPre, const double* dtdev, FusedTile tile,
@@ -975,7 +993,9 @@ void fillArgs(FusedArgs<T>& a, int stage, const Layout& L, const Params& P, cons
     a.L = L;
     a.W = W;
     a.c = makeCoef<T>(P);
-    if (L.fstride * sizeof(T) > 0xFFFFFFFFull) throw std::length_error("field larger than 4 GiB: 32-bit offsets");
+    // byte offsets within a tile (its z-chunk, the plane below and the plane above) are 32-bit
+    if ((size_t)(chunkPlanes(tile) + 2) * L.plane * sizeof(T) > 0xFFFFFFFFull)
+        throw std::length_error("z-chunk of more than 4 GiB of planes: 32-bit tile offsets");
     for (int q = 0; q < 5; ++q) {
         a.Sin[q] = Sin + q * L.fstride;
         a.U[q] = U + q * L.fstride;
@@ -988,7 +1008,7 @@ void fillArgs(FusedArgs<T>& a, int stage, const Layout& L, const Params& P, cons
     a.rhoEPre = rhoEPre;
     a.dtdev = dtdev;
     const int ty = normTY(tile.ty);
-    a.kc = tile.kc > 0 ? tile.kc : 64;
+    a.kc = chunkPlanes(tile);
     a.nbx = (L.NI + 61) / 62;
     a.nby = (L.NJ + ty - 1) / ty;
     a.nbz = (L.NK + a.kc - 1) / a.kc;

@@ -250,3 +250,64 @@ def test_adjoint_objective_fp32_forward_against_fp64():
     rel = abs(J32 - r["J"]) / abs(r["J"])
     print(f"J FP64 {r['J']:.12e} FP32 {J32:.12e} rel {rel:.2e}")
     assert rel <= 1e-5, rel
+
+
+# ------------------------------------------------------------------ fields beyond 4 GiB
+
+@pytest.mark.parametrize("precision,nk", [("fp32", 44500), ("fp64", 29700)])
+def test_field_beyond_4gib_matches_a_thin_slab(precision, nk):
+    """A fully periodic box whose initial field varies in x and y only stays z-uniform exactly (every z-derivative
+    is a difference of equal values), so every plane of a run with more than 4 GiB per field (the fused stage
+    kernel's global-access form, offsets from each tile's first plane) must equal, bit for bit, the plane of the
+    same run on 5 planes (z spacing 1/16 in both, so the step sizes agree too). 3 steps of the default form
+    (FP32: compensated combine)."""
+    dz = 0.0625
+
+    def make(NK):
+        return nsfsi.make_config(
+            NI=126, NJ=126, NK=NK, L_x=4.0, L_y=4.0, L_z=dz * (NK - 2), bc=["periodic"] * 6, body_type="none",
+            stop_criterion="timesteps", stop_time_level=3, filter_intervals=[0], filter_change_times=[],
+            M_0=0.3, Re=300.0, conv_stability_limit=0.5, precision=precision)
+
+    def plane_state(cfg):
+        # x_i = (i - 1) hx on the periodic axes; a smooth field of one period in x and in y
+        hx, hy = cfg.L_x / (cfg.NI - 2), cfg.L_y / (cfg.NJ - 2)
+        x = (np.arange(cfg.NI) - 1) * hx
+        y = (np.arange(cfg.NJ) - 1) * hy
+        X, Y = np.meshgrid(x, y)  # [j, i]
+        sx, sy = np.sin(2 * np.pi * X / cfg.L_x), np.sin(2 * np.pi * Y / cfg.L_y)
+        rho = 0.01 * sx * sy
+        u = 0.3 + 0.05 * np.cos(2 * np.pi * Y / cfg.L_y)
+        v = 0.05 * sx
+        p = 0.002 * np.cos(2 * np.pi * X / cfg.L_x)
+        m = 1 + rho
+        return np.stack([rho, m * u, m * v, np.zeros_like(rho), p / (cfg.gamma - 1) + m / 2 * (u * u + v * v)])
+
+    big, thin = make(nk), make(5)
+    eb = 4 if precision == "fp32" else 8
+    # device row: x-segments of 62 nodes in 64 slots, padded to whole segments (FP32) or 128 bytes (FP64)
+    slots = (big.NI - 1) + 2 * ((big.NI - 1) // 62) + 2
+    unit = 64 if precision == "fp32" else 128 // eb
+    field_bytes = nk * big.NJ * (-(-slots // unit) * unit) * eb
+    print(f"{precision} 126x126x{nk}: {field_bytes / 2**30:.2f} GiB per field", flush=True)
+    assert field_bytes > 2**32
+    p0 = plane_state(thin)
+    t = solver(thin)
+    t.setInitialField(np.broadcast_to(p0[:, None], (5, 5, 126, 126)))
+    t.marchTimeStep(3)
+    ref = [t.mesh.field(nsfsi.FIELDS[q])[2] for q in range(5)]
+    tt = t.time()
+    t.close()
+    s = solver(big, graph=1)
+    U = np.empty((5, nk, 126, 126))
+    U[:] = p0[:, None]
+    s.setInitialField(U)
+    del U
+    s.marchTimeStep(3)
+    assert s.time() == tt
+    for q in range(5):
+        f = s.mesh.field(nsfsi.FIELDS[q])
+        for k in (1, 2, nk // 3, nk // 2, nk - 2):
+            assert np.array_equal(f[k], ref[q]), (nsfsi.FIELDS[q], k)
+        del f
+    s.close()
