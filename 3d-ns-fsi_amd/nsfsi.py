@@ -482,9 +482,12 @@ class Solver:
         self._check(lib().ns_set_dt_schedule(self._h, _dptr(a), a.size))
 
     def inletPlaneShape(self):
-        """(n_b, n_a) shape of the inlet plane's profile array (x inlet: (NK, NJ))."""
+        """(n_b, n_a) shape of the inlet plane's profile array (x inlet: (NK, NJ)); (0, 0) without an inlet."""
         c = self.config
-        axis = [c.bc[0], c.bc[1], c.bc[2], c.bc[3], c.bc[4], c.bc[5]].index(BC["inlet"]) // 2
+        faces = [c.bc[q] for q in range(6)]
+        if BC["inlet"] not in faces:
+            return (0, 0)
+        axis = faces.index(BC["inlet"]) // 2
         return {0: (c.NK, c.NJ), 1: (c.NK, c.NI), 2: (c.NJ, c.NI)}[axis]
 
     def setInflowProfile(self, q):
@@ -506,7 +509,7 @@ class Solver:
         J, gre = C.c_double(), C.c_double()
         dts = np.zeros(nsteps + 1)
         self._check(lib().ns_adjoint_drag(self._h, int(nsteps), int(n0), int(n1), int(checkpoint), C.byref(J),
-                                          _dptr(gu), _dptr(gq), C.byref(gre), _dptr(dts)))
+                                          _dptr(gu), _dptr(gq) if gq.size else None, C.byref(gre), _dptr(dts)))
         c = self.config
         return {"J": J.value, "grad_u0": gu.reshape(5, c.NK, c.NJ, c.NI), "grad_inflow": gq.reshape(shp),
                 "grad_re": gre.value, "dts": dts}

@@ -424,7 +424,11 @@ public:
         double dtVisc = std::fabs(P.visc_stability_limit) / (nuMax * (1 / dx2 + 1 / dy2 + 1 / dz2));
         dt = std::min(dtConv, dtVisc);
         if (P.stop_criterion == 1 && tnow + dt > P.t_end) dt = P.t_end - tnow;
+        // prescribed step sizes (test hook, as the device's ns_set_dt_schedule): the step from time level n
+        // takes dts[n]
+        if (!dtSchedule.empty() && timeLevel < dtSchedule.size()) dt = dtSchedule[timeLevel];
     }
+    std::vector<double> dtSchedule;
 
     // ---------------------------------------------------------------- RK4 slopes, src/Solver.cpp:170-418
     void slopes(const std::array<std::vector<double>, 5>& U, std::array<std::vector<double>, 5>& K) {
@@ -1133,6 +1137,15 @@ int ns_oracle_set_cr_pow(void* h, int on) {
 }
 
 /* Unit hook: x^1.5 as the oracle evaluates it (mode 0 std::pow, 1 correctly rounded). */
+/* Prescribed step sizes: the step from time level n takes dts[n] (n < len); also sets the current dt. */
+int ns_oracle_set_dt_schedule(void* h, const double* dts, int64_t n) {
+    auto* o = (Oracle*)h;
+    if (!o || n < 0) return NS_ERR_ARGUMENT;
+    o->dtSchedule.assign(dts, dts + n);
+    if (o->timeLevel < o->dtSchedule.size()) o->dt = o->dtSchedule[o->timeLevel];
+    return NS_OK;
+}
+
 double ns_oracle_pow15(double x, int mode) { return mode ? Oracle::pow15cr(x) : std::pow(x, 1.5); }
 
 /* Unit hook: 8x8 full-pivot LU solve of A x = b (A row-major). */

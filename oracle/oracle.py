@@ -70,6 +70,7 @@ def lib():
     L.ns_oracle_rho_filter.argtypes = [H]
     L.ns_oracle_norm_of_change.argtypes = [H, C.c_int]
     L.ns_oracle_norm_of_change.restype = C.c_double
+    L.ns_oracle_set_dt_schedule.argtypes = [H, D, C.c_int64]
     L.ns_oracle_update_dt.argtypes = [H, C.c_double]
     L.ns_oracle_update_dt.restype = C.c_double
     _lib = L
@@ -204,6 +205,11 @@ class Oracle:
 
     def norm_of_change(self, q) -> float:
         return lib().ns_oracle_norm_of_change(self._h, int(q))
+
+    def set_dt_schedule(self, dts):
+        """The step from time level n takes dts[n] (the device's ns_set_dt_schedule)."""
+        a = np.ascontiguousarray(dts, dtype=np.float64)
+        self._check(lib().ns_oracle_set_dt_schedule(self._h, _d(a), a.size))
 
     def update_dt(self, tnow) -> float:
         return lib().ns_oracle_update_dt(self._h, float(tnow))

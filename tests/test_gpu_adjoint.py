@@ -166,3 +166,57 @@ def test_fourth_order_differences():
         rel = abs(fd - ad) / abs(ad)
         print(f"4th-order FD={fd:.14e} adjoint={ad:.14e} rel={rel:.2e}", flush=True)
         assert rel <= 5e-9
+
+
+def test_gradient_against_differences_of_the_oracle():
+    """The adjoint gradient against central differences of the serial FP64 oracle, not of the device: in a box
+    periodic in every direction the drag is minus the node volume times the sum of the momentum right-hand
+    sides over the FluidInterior nodes (the reference's own per-node expression, src/Solver.cpp:170-418), so
+    J(U0) = mean C_D over levels n0..n1 is computed from the oracle's state after each of its steps, at the
+    device run's step sizes (prescribed in both). J agrees with the device's to 1e-10, and the adjoint's
+    directional derivatives with the oracle's differences to 1e-6 on three random directions."""
+    import oracle
+
+    cfg = nsfsi.make_config(
+        NI=24, NJ=22, NK=20, L_x=2.0, L_y=2.0, L_z=2.0, bc=["periodic"] * 6, body_type="sphere",
+        body_centre=(1.03, 0.98, 1.01), body_radius=0.42, ic_type="freestream_gaussian", ic_amplitude=1e-3,
+        ic_seed=5, M_0=0.3, Re=300.0, stop_criterion="timesteps", stop_time_level=6, filter_intervals=[0],
+        filter_change_times=[], conv_stability_limit=0.5)
+    nsteps, n0, n1 = 6, 3, 6
+    s = nsfsi.Solver(cfg)
+    s.initialize()
+    U = s.mesh.conserved()
+    s.close()
+    r = nt.drag_and_gradients(cfg, U, None, cfg.Re, nsteps, n0, n1, 3)
+    area = np.pi * cfg.body_radius ** 2
+
+    def J_oracle(U0):
+        o = oracle.Oracle(cfg)
+        o.initialize()
+        for q in range(5):
+            o.set_field(q, U0[q])
+        o.derive_all()
+        o.set_dt_schedule(r["dts"])
+        (sp, _) = o.geometry()
+        cd = []
+        for level in range(1, nsteps + 1):
+            o.march(1)
+            if level >= n0:
+                fx = -sp.prod() * o.slopes()[1].sum()
+                cd.append(fx / (0.5 * cfg.M_0 ** 2 * area))
+        return float(np.mean(cd))
+
+    j0 = J_oracle(U)
+    print(f"J device {r['J']:.15e} oracle {j0:.15e}", flush=True)
+    assert abs(j0 - r["J"]) <= 1e-10 * abs(j0)
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(3):
+        d = rng.standard_normal(U.shape)
+        d /= np.abs(d).max()
+        fd = (J_oracle(U + EPS * d) - J_oracle(U - EPS * d)) / (2 * EPS)
+        ad = float(np.sum(r["grad_u0"] * d))
+        rel = abs(fd - ad) / abs(fd)
+        print(f"  oracle FD={fd:.10e} adjoint={ad:.10e} rel={rel:.2e}", flush=True)
+        worst = max(worst, rel)
+    assert worst <= TOL, worst
