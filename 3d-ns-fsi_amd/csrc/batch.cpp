@@ -29,8 +29,6 @@ Batch::Batch(const ns_config* cfgs, int n) {
         if (k.kahan != cfgs[0].kahan) throw Unsupported("all batched cases need the same kahan setting");
         if (k.n_extra_bodies != 0) throw Unsupported("batched cases hold at most one immersed body");
         cases.push_back(std::make_unique<Case>(k));
-        if (!cases.back()->filterDisabled())
-            throw Unsupported("the periodic density filter is not available in batched runs (filterIntervals = [0])");
     }
     kahan = cfgs[0].kahan != 0;
     hipCheck(hipStreamCreate(&stream), "stream");
@@ -108,6 +106,13 @@ void Batch::init() {
         v.nGhost = k.nGhost;
         v.inactiveList = k.inactiveList;
         v.inactiveIC = k.inactiveIC;
+        v.fs = FilterSchedule{};
+        v.fs.n_intervals = k.cfg.n_filter_intervals;
+        v.fs.n_change = k.cfg.n_filter_change_times;
+        for (int q = 0; q < NS_MAX_FILTER; ++q) {
+            v.fs.intervals[q] = k.cfg.filter_intervals[q];
+            v.fs.change_times[q] = k.cfg.filter_change_times[q];
+        }
         v.nInactive = k.nInactive;
         v.scal = k.scal;
         v.partials = k.partials;
@@ -129,6 +134,22 @@ void Batch::init() {
         ib[c + 1] = ib[c] + v.nInactiveBlocks;
     }
     hostViews = views;
+    // cases with a periodic density filter, and their nodes (x fastest) as one thread range
+    std::vector<int> fids;
+    std::vector<int64_t> fstartN(1, 0);
+    for (int c = 0; c < n; ++c)
+        if (!cases[c]->filterDisabled()) {
+            fids.push_back(c);
+            fstartN.push_back(fstartN.back() + (int64_t)cases[c]->L.nodes());
+        }
+    nFiltCases = (int)fids.size();
+    filtNodeTotal = fstartN.back();
+    if (nFiltCases > 0) {
+        devFiltIds = (int*)alloc(fids.size() * sizeof(int));
+        devFiltStart = (int64_t*)alloc(fstartN.size() * sizeof(int64_t));
+        hipCheck(hipMemcpy(devFiltIds, fids.data(), fids.size() * sizeof(int), hipMemcpyHostToDevice), "copy");
+        hipCheck(hipMemcpy(devFiltStart, fstartN.data(), fstartN.size() * sizeof(int64_t), hipMemcpyHostToDevice), "copy");
+    }
     devViews = (CaseView<float>*)alloc(n * sizeof(CaseView<float>));
     for (int f = 0; f < 6; ++f) {
         devFaceStart[f] = (int64_t*)alloc((n + 1) * sizeof(int64_t));
@@ -190,10 +211,13 @@ void Batch::buildStageTables() {
                 float* A = v.buf[1 - p];
                 float* in[4] = {U, v.buf[2], v.buf[3], v.buf[2]};
                 float* out[4] = {v.buf[2], v.buf[3], v.buf[2], A};
+                // stage 1 of a case with the density filter takes the filtered density from S2 when it fired
+                const bool filt = st == 1 && !k.filterDisabled();
                 int nb = buildFusedArgs(h.data() + c * ab, st, k.L, k.P, k.W, in[st - 1], U, out[st - 1], A, k.cls,
                                         (const float*)k.rhoPre, (const float*)k.rhoEPre, &k.scal->dt, tile,
                                         st == 4 ? k.partials : nullptr, k.cfg.conv_history > 0,
-                                        kahan ? k.comp : nullptr, k.stageOutlet());
+                                        kahan ? k.comp : nullptr, k.stageOutlet(), filt ? v.buf[3] : nullptr,
+                                        filt ? &k.scal->filterNow : nullptr);
                 start[c + 1] = start[c] + nb;
                 hostViews[c].nStage4 = nb;
                 const size_t need = (size_t)nb + hostViews[c].nBoundaryBlocks + hostViews[c].nInactiveBlocks;
@@ -220,8 +244,16 @@ void Batch::stepOnce() {
     const int n = (int)cases.size();
     const int outRole[4] = {kRoleS1, kRoleS2, kRoleS1, kRoleA};
     const double frac[4] = {0.5, 0.5, 1.0, 1.0};
+    // periodic density filter (src/Solver.cpp:134), as the single-case step: decided on the device, the
+    // filtered density in S2 for stage 1, committed to U after it
+    if (nFiltCases > 0) {
+        launchFilterFlagBatched(stream, devViews, devFiltIds, nFiltCases);
+        launchRhoFilterBatched(stream, devViews, devFiltIds, devFiltStart, nFiltCases, filtNodeTotal, parity);
+    }
     for (int st = 1; st <= 4; ++st) {
         launchStageBatched(stream, st, tile.ty, true, kahan, devArgs[parity][st - 1], devStageStart, n, totalTiles);
+        if (st == 1 && nFiltCases > 0)
+            launchRhoCommitBatched(stream, devViews, devFiltIds, devFiltStart, nFiltCases, filtNodeTotal, parity);
         applyBCs(outRole[st - 1], frac[st - 1]);
     }
     launchBoundaryReduceBatched<float>(stream, devViews, devBBStart, totalBB, n, parity);

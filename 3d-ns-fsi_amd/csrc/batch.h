@@ -43,6 +43,7 @@ struct CaseView {
     double tEnd, cfl, sigma;
     int nStage4, nBoundaryBlocks, nInactiveBlocks;
     int norms;
+    FilterSchedule fs;      // periodic density filter (src/Mesh.cpp:85-140), decided on the device
 };
 
 template <class T>
@@ -63,6 +64,14 @@ __device__ __forceinline__ int findCase(const I* __restrict__ start, int ncases,
 }
 
 // ---- batched launchers (one launch for all cases; *Start arrays are device prefix sums over cases)
+// Periodic density filter of the cases ids[0..n) (those with a filter interval): the schedule test, the
+// filtered density into each case's S2 (nodes over the cases by the prefix sums start), its commit to U
+// after stage 1 (stage_plain.hip).
+void launchFilterFlagBatched(hipStream_t s, const CaseView<float>* cv, const int* ids, int n);
+void launchRhoFilterBatched(hipStream_t s, const CaseView<float>* cv, const int* ids, const int64_t* start, int n,
+                            int64_t total, int parity);
+void launchRhoCommitBatched(hipStream_t s, const CaseView<float>* cv, const int* ids, const int64_t* start, int n,
+                            int64_t total, int parity);
 // Faces of every case launched together (faces that are independent in every case): member m is face
 // face[m] of each case, its nodes over the cases given by the device prefix sums start[m], and it takes
 // the thread range [off[m], off[m+1]).

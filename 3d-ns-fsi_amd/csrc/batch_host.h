@@ -56,6 +56,10 @@ private:
     int* devBBStart = nullptr;
     int* devIBStart = nullptr;
     int totalBB = 0, totalIB = 0;
+    int nFiltCases = 0;  // cases with a periodic density filter
+    int* devFiltIds = nullptr;
+    int64_t* devFiltStart = nullptr;
+    int64_t filtNodeTotal = 0;
     hipGraphExec_t graphExec[2] = {nullptr, nullptr};
     std::vector<void*> allocations;
 };

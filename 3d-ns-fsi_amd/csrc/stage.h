@@ -97,7 +97,8 @@ size_t fusedArgsBytes();
 int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const Window& W, const float* Sin,
                    const float* U, float* Sout, float* A, const uint8_t* cls, const float* rhoPre, const float* rhoEPre,
                    const double* dtdev, FusedTile tile, double* partials, int norms, int8_t* comp,
-                   const EdgeFace* outlet = nullptr);
+                   const EdgeFace* outlet = nullptr, const float* filteredRho = nullptr,
+                   const int32_t* filterFlag = nullptr);
 void launchStageBatched(hipStream_t s, int stage, int ty, bool scan, bool kahan, const void* tab, const int* start,
                         int ncases, int total);
 // Force on the body from the state S after a step's boundary application (FP64; the discrete momentum

@@ -1049,11 +1049,11 @@ size_t fusedArgsBytes() { return sizeof(FusedArgs<float>); }
 int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const Window& W, const float* Sin,
                    const float* U, float* Sout, float* A, const uint8_t* cls, const float* rhoPre, const float* rhoEPre,
                    const double* dtdev, FusedTile tile, double* partials, int norms, int8_t* comp,
-                   const EdgeFace* outlet) {
+                   const EdgeFace* outlet, const float* filteredRho, const int32_t* filterFlag) {
     if (!stateFitsBuffer(L)) throw std::length_error("batched case: state buffer beyond 4 GiB");
     FusedArgs<float> a;
-    fillArgs<float>(a, stage, L, P, W, Sin, U, Sout, A, cls, rhoPre, rhoEPre, dtdev, tile, partials, norms, nullptr,
-                    nullptr, comp, outlet);
+    fillArgs<float>(a, stage, L, P, W, Sin, U, Sout, A, cls, rhoPre, rhoEPre, dtdev, tile, partials, norms, filteredRho,
+                    filterFlag, comp, outlet);
     std::memcpy(dst, &a, sizeof a);
     return a.nblocks;
 }

@@ -21,6 +21,7 @@
 
 #include <stdexcept>
 
+#include "batch.h"
 #include "device_common.h"
 #include "stage.h"
 
@@ -303,7 +304,7 @@ void launchRhsPlain(hipStream_t s, int stage, const Layout& L, const Params& P, 
 
 namespace {
 
-__global__ void k_filter_flag(FilterSchedule fs, StepScalars* st) {
+__device__ __forceinline__ void filterFlagOf(const FilterSchedule& fs, StepScalars* st) {
     // Mesh::checkFilterCondition, src/Mesh.cpp:85-103, with t and timeLevel at the start of the step
     int counter = 0;
     for (int q = 0; q < fs.n_change; ++q)
@@ -316,11 +317,10 @@ __global__ void k_filter_flag(FilterSchedule fs, StepScalars* st) {
     st->filterNow = now;
 }
 
+__global__ void k_filter_flag(FilterSchedule fs, StepScalars* st) { filterFlagOf(fs, st); }
+
 template <class T>
-__global__ void k_rho_filter(Layout L, const T* U, T* out, const uint8_t* cls, const int32_t* flag) {
-    if (!*flag) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-    if (i >= L.NI) return;
+__device__ __forceinline__ void rhoFilterNode(const Layout& L, const T* U, T* out, const uint8_t* cls, int i, int j, int k) {
     size_t o = L.off(i, j, k);
     if ((cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) return;
     const long XP = Layout::xstep(i, 1), XM = Layout::xstep(i, -1), Y = L.pitch, Z = (long)L.plane;
@@ -329,14 +329,67 @@ __global__ void k_rho_filter(Layout L, const T* U, T* out, const uint8_t* cls, c
 }
 
 template <class T>
-__global__ void k_rho_commit(Layout L, T* U, const T* filtered, const T* A, const uint8_t* cls, const int32_t* flag) {
-    if (!*flag) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-    if (i >= L.NI) return;
+__device__ __forceinline__ void rhoCommitNode(const Layout& L, T* U, const T* filtered, const T* A, const uint8_t* cls,
+                                              int i, int j, int k) {
     size_t o = L.off(i, j, k);
     uint8_t c = cls[o];
     if ((c & CLS_TYPE_MASK) == NS_FLUID_INTERIOR) U[o] = filtered[o];
     else if (c & CLS_INACTIVE_IC) U[o] = A[o];  // the swapped-in scratch array keeps its solid values
+}
+
+template <class T>
+__global__ void k_rho_filter(Layout L, const T* U, T* out, const uint8_t* cls, const int32_t* flag) {
+    if (!*flag) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.NI) return;
+    rhoFilterNode(L, U, out, cls, i, j, k);
+}
+
+template <class T>
+__global__ void k_rho_commit(Layout L, T* U, const T* filtered, const T* A, const uint8_t* cls, const int32_t* flag) {
+    if (!*flag) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.NI) return;
+    rhoCommitNode(L, U, filtered, A, cls, i, j, k);
+}
+
+// ---- batched forms: only the cases whose filter is on (ids[m], nodes over them by the prefix sums start)
+__global__ void k_filter_flag_batched(const CaseView<float>* __restrict__ cv, const int* __restrict__ ids, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) filterFlagOf(cv[ids[m]].fs, cv[ids[m]].scal);
+}
+
+__device__ __forceinline__ void batchNode(const CaseView<float>& v, int64_t r, int& i, int& j, int& k) {
+    i = (int)(r % v.L.NI);
+    const int64_t q = r / v.L.NI;
+    j = (int)(q % v.L.NJ);
+    k = (int)(q / v.L.NJ);
+}
+
+// the filtered density goes into S2's density field (free until stage 2), as in the single-case step
+__global__ void k_rho_filter_batched(const CaseView<float>* __restrict__ cv, const int* __restrict__ ids,
+                                     const int64_t* __restrict__ start, int n, int parity) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= start[n]) return;
+    const int m = findCase(start, n, t);
+    const CaseView<float>& v = cv[ids[m]];
+    if (!v.scal->filterNow) return;
+    int i, j, k;
+    batchNode(v, t - start[m], i, j, k);
+    rhoFilterNode(v.L, (const float*)roleBuffer(v, 0, parity), v.buf[3], v.cls, i, j, k);
+}
+
+__global__ void k_rho_commit_batched(const CaseView<float>* __restrict__ cv, const int* __restrict__ ids,
+                                     const int64_t* __restrict__ start, int n, int parity) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= start[n]) return;
+    const int m = findCase(start, n, t);
+    const CaseView<float>& v = cv[ids[m]];
+    if (!v.scal->filterNow) return;
+    int i, j, k;
+    batchNode(v, t - start[m], i, j, k);
+    rhoCommitNode(v.L, roleBuffer(v, 0, parity), (const float*)v.buf[3], (const float*)roleBuffer(v, 1, parity),
+                  v.cls, i, j, k);
 }
 
 }  // namespace
@@ -356,6 +409,24 @@ void launchRhoCommit(hipStream_t s, const Layout& L, T* U, const T* filtered, co
                      const int32_t* flag) {
     dim3 blk(64), grd((L.NI + 63) / 64, L.NJ, L.NK);
     hipLaunchKernelGGL(k_rho_commit<T>, grd, blk, 0, s, L, U, filtered, A, cls, flag);
+}
+
+void launchFilterFlagBatched(hipStream_t s, const CaseView<float>* cv, const int* ids, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_filter_flag_batched, dim3((n + 63) / 64), dim3(64), 0, s, cv, ids, n);
+}
+
+void launchRhoFilterBatched(hipStream_t s, const CaseView<float>* cv, const int* ids, const int64_t* start, int n,
+                            int64_t total, int parity) {
+    if (total > 0)
+        hipLaunchKernelGGL(k_rho_filter_batched, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, cv, ids, start, n,
+                           parity);
+}
+
+void launchRhoCommitBatched(hipStream_t s, const CaseView<float>* cv, const int* ids, const int64_t* start, int n,
+                            int64_t total, int parity) {
+    if (total > 0)
+        hipLaunchKernelGGL(k_rho_commit_batched, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, cv, ids, start, n,
+                           parity);
 }
 
 template void launchRhoFilter<double>(hipStream_t, const Layout&, const double*, double*, const uint8_t*, const int32_t*);

@@ -109,3 +109,39 @@ def test_batch_plain_combine():
         U, _, _ = single(cfg, 4, tile)
         assert np.array_equal(b.conserved(c), U)
     b.close()
+
+
+@pytest.mark.parametrize("graph", [1, 0])
+def test_batch_with_periodic_density_filter(graph):
+    """Cases with the periodic density filter (src/Mesh.cpp:85-140) in a batch beside cases without: the
+    shipped 161x83x3 case (intervals 50/200/1000 switching at t = 30/60/100; fires at step 50) and a 24^3
+    sphere filtered every 3 steps, 60 steps; every case equals its single-case run bit for bit."""
+    steps = 60
+    ref = nsfsi.make_config(precision="fp32", stop_criterion="timesteps", stop_time_level=steps)
+    sph = cases.sphere48(N=24, steps=steps, precision="fp32")
+    sph.n_filter_intervals = 1
+    sph.filter_intervals[0] = 3
+    sph.n_filter_change_times = 0
+    plain = cases.sweep64(n=1, steps=steps)[0]
+    cfgs = [ref, plain, sph]
+    for c in cfgs:
+        c.kahan = 1
+    print("batch with filters, graph", graph, flush=True)
+    b = nsfsi.Batch(cfgs)
+    b.initialize()
+    b.setOption("graph", graph)
+    tile = b.tile()
+    b.step(steps)
+    for c, cfg in enumerate(cfgs):
+        U, (t, lv, dt), _ = single(cfg, steps, tile)
+        bt, blv, bdt = b.time(c)
+        assert blv == lv == steps and bt == t and bdt == dt, c
+        assert np.array_equal(b.conserved(c), U), c
+    filtered = b.conserved(0)
+    b.close()
+    # the filter changed the run: without it the shipped case ends elsewhere
+    nofilter = nsfsi.make_config(precision="fp32", stop_criterion="timesteps", stop_time_level=steps,
+                                 filter_intervals=[0], filter_change_times=[])
+    nofilter.kahan = 1
+    U0, _, _ = single(nofilter, steps, tile)
+    assert not np.array_equal(U0, filtered)
