@@ -506,9 +506,9 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     if (NB == 1) __syncthreads();  // the readers of plane kb-1 are done before its buffer is rewritten
     finishPlane<T, TY>(a, smb[NB - 1], row, lane, rnext, q0, inner);
     rnext = fetch(a, m, ic, jc, colc, kb + 1, last);
-    // The stencil input is fetched two planes ahead of its use, except in the FP32 stage 4, whose
-    // registers go to the scans and the compensation (two planes ahead spill there).
-    constexpr bool kDeep = STAGE != 4 || sizeof(T) == 8;
+    // The stencil input is fetched two planes ahead of its use, except in the FP32 stage 4 with global
+    // addressing, whose 64-bit addresses, scans and compensation leave no registers for it (it spills).
+    constexpr bool kDeep = STAGE != 4 || sizeof(T) == 8 || BUF;
     Raw<T> rnext2{};
     if constexpr (kDeep) rnext2 = fetch(a, m, ic, jc, colc, kb + 2, last);
     T gz0[5], phizm[5];
