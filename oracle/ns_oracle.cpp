@@ -9,6 +9,10 @@
  * floating-point expression order, and must be built like the reference Release build: g++ -O3, no
  * -march, -ffp-contract=off (no FMA contraction).
  *
+ * The class is a template over the scalar type of the state: on double it is the oracle; on Dual (value,
+ * tangent) it is the oracle's tangent-linear form, the forward-mode derivative of the same step at given
+ * step sizes (ns_oracle_tl_*), used to check the device adjoint's gradients.
+ *
  * Where the reference calls Eigen's FullPivLU (src/Boundary.cpp:825), this file has its own textbook
  * full-pivot LU (lu8_*); agreement with Eigen's own rounding is parity unpinned. The sphere body uses
  * the cylinder's ghost-augmentation loop (src/Boundary.cpp:912-916): the sphere's own loop
@@ -26,6 +30,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../include/ns_fsi.h"
@@ -55,9 +60,9 @@ struct Box {
     size_t iMin, iMax, jMin, jMax, kMin, kMax;
 };
 
-struct Cons { double rho, ru, rv, rw, rE; };
-struct Prim { double u, v, w, p, T; };
-struct Trans { double mu, kappa; };
+template <class R> struct ConsT { R rho, ru, rv, rw, rE; };
+template <class R> struct PrimT { R u, v, w, p, T; };
+template <class R> struct TransT { R mu, kappa; };
 
 struct Ghost {
     U3 idx;
@@ -74,9 +79,39 @@ static inline double uniform01(uint64_t seed, uint64_t counter) {
     return (double)(mix64(seed + 0x9E3779B97F4A7C15ULL * (counter + 1)) >> 11) * 0x1.0p-53;
 }
 
-class Oracle {
+// Value and tangent of one quantity: the oracle instantiated on this type is the tangent-linear form of
+// the step (forward-mode differentiation). Each value is computed by exactly the operations the plain
+// FP64 oracle applies to it, so the values of a tangent run equal the plain run bit for bit; the tangents
+// follow the sum, product and quotient rules (and d(x^1.5) = 1.5 sqrt(x) dx). Step sizes, norms of change
+// and the time stay plain values: the tangent model is that of the step at given step sizes.
+struct Dual {
+    double v = 0, d = 0;
+    Dual() = default;
+    Dual(double x) : v(x) {}
+    Dual(double x, double dx) : v(x), d(dx) {}
+    Dual& operator+=(const Dual& b) { v += b.v; d += b.d; return *this; }
+    Dual& operator-=(const Dual& b) { v -= b.v; d -= b.d; return *this; }
+    Dual& operator/=(const Dual& b) { const double q = v / b.v; d = (d - q * b.d) / b.v; v = q; return *this; }
+};
+static inline Dual operator-(const Dual& a) { return {-a.v, -a.d}; }
+static inline Dual operator+(const Dual& a, const Dual& b) { return {a.v + b.v, a.d + b.d}; }
+static inline Dual operator-(const Dual& a, const Dual& b) { return {a.v - b.v, a.d - b.d}; }
+static inline Dual operator*(const Dual& a, const Dual& b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+static inline Dual operator/(const Dual& a, const Dual& b) {
+    const double q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+static inline double valueOf(double x) { return x; }
+static inline double valueOf(const Dual& x) { return x.v; }
+
+template <class R>
+class OracleT {
 public:
-    explicit Oracle(const ns_config& c) : P(c) {
+    using Cons = ConsT<R>;
+    using Prim = PrimT<R>;
+    using Trans = TransT<R>;
+
+    explicit OracleT(const ns_config& c) : P(c) {
         NI = (size_t)P.NI; NJ = (size_t)P.NJ; NK = (size_t)P.NK;
         if (P.NI < 3 || P.NJ < 3 || P.NK < 3) throw std::invalid_argument("NI, NJ, NK must be >= 3");
         N = NI * NJ * NK;
@@ -118,15 +153,15 @@ public:
 
     // ---------------------------------------------------------------- EOS, src/FlowVariableGroupStructs.h:152-185
     Cons deriveCons(const Prim& q) const {
-        double rho = (P.gamma * q.p - q.T) / (1 + q.T);
-        double ru = (1 + rho) * q.u, rv = (1 + rho) * q.v, rw = (1 + rho) * q.w;
-        double rE = q.p / (P.gamma - 1) + (1 + rho) / 2 * (q.u * q.u + q.v * q.v + q.w * q.w);
+        R rho = (P.gamma * q.p - q.T) / (1 + q.T);
+        R ru = (1 + rho) * q.u, rv = (1 + rho) * q.v, rw = (1 + rho) * q.w;
+        R rE = q.p / (P.gamma - 1) + (1 + rho) / 2 * (q.u * q.u + q.v * q.v + q.w * q.w);
         return {rho, ru, rv, rw, rE};
     }
     Prim derivePrim(const Cons& c) const {
-        double u = c.ru / (1 + c.rho), v = c.rv / (1 + c.rho), w = c.rw / (1 + c.rho);
-        double p = (P.gamma - 1) * (c.rE - (1 + c.rho) / 2 * (u * u + v * v + w * w));
-        double T = (P.gamma * p - c.rho) / (1 + c.rho);
+        R u = c.ru / (1 + c.rho), v = c.rv / (1 + c.rho), w = c.rw / (1 + c.rho);
+        R p = (P.gamma - 1) * (c.rE - (1 + c.rho) / 2 * (u * u + v * v + w * w));
+        R T = (P.gamma * p - c.rho) / (1 + c.rho);
         return {u, v, w, p, T};
     }
     // x^1.5 (src/Solver.cpp:476, src/FlowVariableGroupStructs.h:183): the reference's std::pow by default;
@@ -141,11 +176,15 @@ public:
         const double pe = std::fma(x, s, -p);
         return p + (pe + x * e);
     }
-    double pow15(double x) const { return crPow ? pow15cr(x) : std::pow(x, 1.5); }
+    // tangent of x^1.5: 1.5 sqrt(x) dx
+    R pow15(const R& x) const {
+        if constexpr (std::is_same<R, double>::value) return crPow ? pow15cr(x) : std::pow(x, 1.5);
+        else return Dual(crPow ? pow15cr(x.v) : std::pow(x.v, 1.5), 1.5 * std::sqrt(x.v) * x.d);
+    }
     Trans deriveTrans(const Prim& q) const {
         double ScPlusOne = 1 + P.sutherlands_C2 / P.T_0;
-        double mu = pow15(1 + q.T) * ScPlusOne / (P.Re * (q.T + ScPlusOne));
-        double kappa = mu / ((P.gamma - 1) * P.Pr);
+        R mu = pow15(1 + q.T) * ScPlusOne / (P.Re * (q.T + ScPlusOne));
+        R kappa = mu / ((P.gamma - 1) * P.Pr);
         return {mu, kappa};
     }
     // setFlowVariablesAtNode, src/FlowVariableGroupStructs.h:189-207 (on the current arrays)
@@ -411,15 +450,15 @@ public:
     void updateTimeStepSize(double tnow) {
         double sx = 0, sy = 0, sz = 0;
         for (size_t n = 0; n < N; ++n) {
-            double c = std::sqrt((1 + P.gamma * prim[3][n]) / (1 + cons[0][n]));
-            sx = std::max(sx, c + std::fabs(prim[0][n]));
-            sy = std::max(sy, c + std::fabs(prim[1][n]));
-            sz = std::max(sz, c + std::fabs(prim[2][n]));
+            double c = std::sqrt((1 + P.gamma * valueOf(prim[3][n])) / (1 + valueOf(cons[0][n])));
+            sx = std::max(sx, c + std::fabs(valueOf(prim[0][n])));
+            sy = std::max(sy, c + std::fabs(valueOf(prim[1][n])));
+            sz = std::max(sz, c + std::fabs(valueOf(prim[2][n])));
         }
         double dtConv = std::fabs(P.conv_stability_limit) / (sx / h.x + sy / h.y + sz / h.z);
         double vm = std::max(4. / 3, P.gamma / P.Pr);
         double nuMax = 0;
-        for (size_t n = 0; n < N; ++n) nuMax = std::max(nuMax, trans[0][n] / (cons[0][n] + 1) * vm);
+        for (size_t n = 0; n < N; ++n) nuMax = std::max(nuMax, valueOf(trans[0][n]) / (valueOf(cons[0][n]) + 1) * vm);
         double dx2 = h.x * h.x, dy2 = h.y * h.y, dz2 = h.z * h.z;
         double dtVisc = std::fabs(P.visc_stability_limit) / (nuMax * (1 / dx2 + 1 / dy2 + 1 / dz2));
         dt = std::min(dtConv, dtVisc);
@@ -431,7 +470,7 @@ public:
     std::vector<double> dtSchedule;
 
     // ---------------------------------------------------------------- RK4 slopes, src/Solver.cpp:170-418
-    void slopes(const std::array<std::vector<double>, 5>& U, std::array<std::vector<double>, 5>& K) {
+    void slopes(const std::array<std::vector<R>, 5>& U, std::array<std::vector<R>, 5>& K) {
         const double dx = h.x, dy = h.y, dz = h.z;
         // node-invariant factors (src/Solver.cpp:202-211, 249-258, 296-305, 345-360)
         const double n2dx = -1. / (2 * dx), n2dy = -1. / (2 * dy), n2dz = -1. / (2 * dz);
@@ -451,85 +490,85 @@ public:
             const size_t e = n + sI, wq = n - sI, nn = n + sJ, s = n - sJ, up = n + sK, d = n - sK;
             // continuity, src/Solver.cpp:191-194
             {
-                double fx = -(ru[e] - ru[wq]) / (2 * dx);
-                double fy = -(rv[nn] - rv[s]) / (2 * dy);
-                double fz = -(rw[up] - rw[d]) / (2 * dz);
+                R fx = -(ru[e] - ru[wq]) / (2 * dx);
+                R fy = -(rv[nn] - rv[s]) / (2 * dy);
+                R fz = -(rw[up] - rw[d]) / (2 * dz);
                 K[0][n] = fx + fy + fz;
             }
             // x-momentum, src/Solver.cpp:225-241
             {
-                double cX = n2dx * (ru[e] * u[e] + p[e] - ru[wq] * u[wq] - p[wq]);
-                double cY = n2dy * (ru[nn] * v[nn] - ru[s] * v[s]);
-                double cZ = n2dz * (ru[up] * w[up] - ru[d] * w[d]);
-                double vXX = p23xx * ((mu[e] + mu[n]) * (u[e] - u[n]) - (mu[wq] + mu[n]) * (u[n] - u[wq]));
-                double vYY = p12yy * ((mu[nn] + mu[n]) * (u[nn] - u[n]) - (mu[s] + mu[n]) * (u[n] - u[s]));
-                double vZZ = p12zz * ((mu[up] + mu[n]) * (u[up] - u[n]) - (mu[d] + mu[n]) * (u[n] - u[d]));
-                double vXY = p14xdy * (mu[nn] * (v[nn + sI] - v[nn - sI]) - mu[s] * (v[s + sI] - v[s - sI]));
-                double vYX = n16ydx * (mu[e] * (v[e + sJ] - v[e - sJ]) - mu[wq] * (v[wq + sJ] - v[wq - sJ]));
-                double vXZ = p14xdz * (mu[up] * (w[up + sI] - w[up - sI]) - mu[d] * (w[d + sI] - w[d - sI]));
-                double vZX = n16zdx * (mu[e] * (w[e + sK] - w[e - sK]) - mu[wq] * (w[wq + sK] - w[wq - sK]));
+                R cX = n2dx * (ru[e] * u[e] + p[e] - ru[wq] * u[wq] - p[wq]);
+                R cY = n2dy * (ru[nn] * v[nn] - ru[s] * v[s]);
+                R cZ = n2dz * (ru[up] * w[up] - ru[d] * w[d]);
+                R vXX = p23xx * ((mu[e] + mu[n]) * (u[e] - u[n]) - (mu[wq] + mu[n]) * (u[n] - u[wq]));
+                R vYY = p12yy * ((mu[nn] + mu[n]) * (u[nn] - u[n]) - (mu[s] + mu[n]) * (u[n] - u[s]));
+                R vZZ = p12zz * ((mu[up] + mu[n]) * (u[up] - u[n]) - (mu[d] + mu[n]) * (u[n] - u[d]));
+                R vXY = p14xdy * (mu[nn] * (v[nn + sI] - v[nn - sI]) - mu[s] * (v[s + sI] - v[s - sI]));
+                R vYX = n16ydx * (mu[e] * (v[e + sJ] - v[e - sJ]) - mu[wq] * (v[wq + sJ] - v[wq - sJ]));
+                R vXZ = p14xdz * (mu[up] * (w[up + sI] - w[up - sI]) - mu[d] * (w[d + sI] - w[d - sI]));
+                R vZX = n16zdx * (mu[e] * (w[e + sK] - w[e - sK]) - mu[wq] * (w[wq + sK] - w[wq - sK]));
                 K[1][n] = cX + cY + cZ + vXX + vYY + vZZ + vXY + vYX + vXZ + vZX;
             }
             // y-momentum, src/Solver.cpp:272-288
             {
-                double cX = n2dx * (rv[e] * u[e] - rv[wq] * u[wq]);
-                double cY = n2dy * (rv[nn] * v[nn] + p[nn] - rv[s] * v[s] - p[s]);
-                double cZ = n2dz * (rv[up] * w[up] - rv[d] * w[d]);
-                double vXX = p12xx * ((mu[e] + mu[n]) * (v[e] - v[n]) - (mu[wq] + mu[n]) * (v[n] - v[wq]));
-                double vYY = p23yy * ((mu[nn] + mu[n]) * (v[nn] - v[n]) - (mu[s] + mu[n]) * (v[n] - v[s]));
-                double vZZ = p12zz * ((mu[up] + mu[n]) * (v[up] - v[n]) - (mu[d] + mu[n]) * (v[n] - v[d]));
-                double vYX = p14ydx * (mu[e] * (u[e + sJ] - u[e - sJ]) - mu[wq] * (u[wq + sJ] - u[wq - sJ]));
-                double vXY = n16xdy * (mu[nn] * (u[nn + sI] - u[nn - sI]) - mu[s] * (u[s + sI] - u[s - sI]));
-                double vYZ = p14ydz * (mu[up] * (w[up + sJ] - w[up - sJ]) - mu[d] * (w[d + sJ] - w[d - sJ]));
-                double vZY = n16zdy * (mu[nn] * (w[nn + sK] - w[nn - sK]) - mu[s] * (w[s + sK] - w[s - sK]));
+                R cX = n2dx * (rv[e] * u[e] - rv[wq] * u[wq]);
+                R cY = n2dy * (rv[nn] * v[nn] + p[nn] - rv[s] * v[s] - p[s]);
+                R cZ = n2dz * (rv[up] * w[up] - rv[d] * w[d]);
+                R vXX = p12xx * ((mu[e] + mu[n]) * (v[e] - v[n]) - (mu[wq] + mu[n]) * (v[n] - v[wq]));
+                R vYY = p23yy * ((mu[nn] + mu[n]) * (v[nn] - v[n]) - (mu[s] + mu[n]) * (v[n] - v[s]));
+                R vZZ = p12zz * ((mu[up] + mu[n]) * (v[up] - v[n]) - (mu[d] + mu[n]) * (v[n] - v[d]));
+                R vYX = p14ydx * (mu[e] * (u[e + sJ] - u[e - sJ]) - mu[wq] * (u[wq + sJ] - u[wq - sJ]));
+                R vXY = n16xdy * (mu[nn] * (u[nn + sI] - u[nn - sI]) - mu[s] * (u[s + sI] - u[s - sI]));
+                R vYZ = p14ydz * (mu[up] * (w[up + sJ] - w[up - sJ]) - mu[d] * (w[d + sJ] - w[d - sJ]));
+                R vZY = n16zdy * (mu[nn] * (w[nn + sK] - w[nn - sK]) - mu[s] * (w[s + sK] - w[s - sK]));
                 K[2][n] = cX + cY + cZ + vXX + vYY + vZZ + vYX + vXY + vYZ + vZY;
             }
             // z-momentum, src/Solver.cpp:320-337
             {
-                double cX = n2dx * (rw[e] * u[e] - rw[wq] * u[wq]);
-                double cY = n2dy * (rw[nn] * v[nn] - rw[s] * v[s]);
-                double cZ = n2dz * (rw[up] * w[up] + p[up] - rw[d] * w[d] - p[d]);
-                double vXX = p12xx * ((mu[e] + mu[n]) * (w[e] - w[n]) - (mu[wq] + mu[n]) * (w[n] - w[wq]));
-                double vYY = p12yy * ((mu[nn] + mu[n]) * (w[nn] - w[n]) - (mu[s] + mu[n]) * (w[n] - w[s]));
-                double vZZ = p23zz * ((mu[up] + mu[n]) * (w[up] - w[n]) - (mu[d] + mu[n]) * (w[n] - w[d]));
-                double vZX = p14zdx * (mu[e] * (u[e + sK] - u[e - sK]) - mu[wq] * (u[wq + sK] - u[wq - sK]));
-                double vXZ = n16xdz * (mu[up] * (u[up + sI] - u[up - sI]) - mu[d] * (u[d + sI] - u[d - sI]));
-                double vZY = p14zdy * (mu[nn] * (v[nn + sK] - v[nn - sK]) - mu[s] * (v[s + sK] - v[s - sK]));
-                double vYZ = n16ydz * (mu[up] * (v[up + sJ] - v[up - sJ]) - mu[d] * (v[d + sJ] - v[d - sJ]));
+                R cX = n2dx * (rw[e] * u[e] - rw[wq] * u[wq]);
+                R cY = n2dy * (rw[nn] * v[nn] - rw[s] * v[s]);
+                R cZ = n2dz * (rw[up] * w[up] + p[up] - rw[d] * w[d] - p[d]);
+                R vXX = p12xx * ((mu[e] + mu[n]) * (w[e] - w[n]) - (mu[wq] + mu[n]) * (w[n] - w[wq]));
+                R vYY = p12yy * ((mu[nn] + mu[n]) * (w[nn] - w[n]) - (mu[s] + mu[n]) * (w[n] - w[s]));
+                R vZZ = p23zz * ((mu[up] + mu[n]) * (w[up] - w[n]) - (mu[d] + mu[n]) * (w[n] - w[d]));
+                R vZX = p14zdx * (mu[e] * (u[e + sK] - u[e - sK]) - mu[wq] * (u[wq + sK] - u[wq - sK]));
+                R vXZ = n16xdz * (mu[up] * (u[up + sI] - u[up - sI]) - mu[d] * (u[d + sI] - u[d - sI]));
+                R vZY = p14zdy * (mu[nn] * (v[nn + sK] - v[nn - sK]) - mu[s] * (v[s + sK] - v[s - sK]));
+                R vYZ = n16ydz * (mu[up] * (v[up + sJ] - v[up - sJ]) - mu[d] * (v[d + sJ] - v[d - sJ]));
                 K[3][n] = cX + cY + cZ + vXX + vYY + vZZ + vZX + vXZ + vZY + vYZ;
             }
             // energy, src/Solver.cpp:378-416
             {
-                const double muP = mu[n], muW = mu[wq], muE = mu[e], muS = mu[s], muN = mu[nn], muD = mu[d], muU = mu[up];
-                const double uP = u[n], uW = u[wq], uE = u[e], uS = u[s], uN = u[nn], uD = u[d], uU = u[up];
-                const double vP = v[n], vW = v[wq], vE = v[e], vS = v[s], vN = v[nn], vD = v[d], vU = v[up];
-                const double wP = w[n], wW = w[wq], wE = w[e], wS = w[s], wN = w[nn], wD = w[d], wU = w[up];
-                const double kP = ka[n], TP = T[n];
-                double inv = n2dx * ((rhoH0 + E[e] + p[e]) * uE - (rhoH0 + E[wq] + p[wq]) * uW)
+                const R muP = mu[n], muW = mu[wq], muE = mu[e], muS = mu[s], muN = mu[nn], muD = mu[d], muU = mu[up];
+                const R uP = u[n], uW = u[wq], uE = u[e], uS = u[s], uN = u[nn], uD = u[d], uU = u[up];
+                const R vP = v[n], vW = v[wq], vE = v[e], vS = v[s], vN = v[nn], vD = v[d], vU = v[up];
+                const R wP = w[n], wW = w[wq], wE = w[e], wS = w[s], wN = w[nn], wD = w[d], wU = w[up];
+                const R kP = ka[n], TP = T[n];
+                R inv = n2dx * ((rhoH0 + E[e] + p[e]) * uE - (rhoH0 + E[wq] + p[wq]) * uW)
                            + n2dy * ((rhoH0 + E[nn] + p[nn]) * vN - (rhoH0 + E[s] + p[s]) * vS)
                            + n2dz * ((rhoH0 + E[up] + p[up]) * wU - (rhoH0 + E[d] + p[d]) * wD);
-                double vx = p23xx * ((muE * uE + muP * uP) * (uE - uP) - (muW * uW + muP * uP) * (uP - uW))
+                R vx = p23xx * ((muE * uE + muP * uP) * (uE - uP) - (muW * uW + muP * uP) * (uP - uW))
                           + p12xx * ((muE * vE + muP * vP) * (vE - vP) - (muW * vW + muP * vP) * (vP - vW))
                           + p12xx * ((muE * wE + muP * wP) * (wE - wP) - (muW * wW + muP * wP) * (wP - wW))
                           + eA * (muE * uE * (v[e + sJ] - v[e - sJ]) - muW * uW * (v[wq + sJ] - v[wq - sJ]))
                           + eB * (muE * uE * (w[e + sK] - w[e - sK]) - muW * uW * (w[wq + sK] - w[wq - sK]))
                           + eC * (muE * vE * (u[e + sJ] - u[e - sJ]) - muW * vW * (u[wq + sJ] - u[wq - sJ]))
                           + eD * (muE * wE * (u[e + sK] - u[e - sK]) - muW * wW * (u[wq + sK] - u[wq - sK]));
-                double vy = p12yy * ((muN * uN + muP * uP) * (uN - uP) - (muS * uS + muP * uP) * (uP - uS))
+                R vy = p12yy * ((muN * uN + muP * uP) * (uN - uP) - (muS * uS + muP * uP) * (uP - uS))
                           + p23yy * ((muN * vN + muP * vP) * (vN - vP) - (muS * vS + muP * vP) * (vP - vS))
                           + p12yy * ((muN * wN + muP * wP) * (wN - wP) - (muS * wS + muP * wP) * (wP - wS))
                           + eA * (muN * vN * (u[nn + sI] - u[nn - sI]) - muS * vS * (u[s + sI] - u[s - sI]))
                           + eE * (muN * vN * (w[nn + sK] - w[nn - sK]) - muS * vS * (w[s + sK] - w[s - sK]))
                           + eC * (muN * uN * (v[nn + sI] - v[nn - sI]) - muS * uS * (v[s + sI] - v[s - sI]))
                           + eF * (muN * wN * (v[nn + sK] - v[nn - sK]) - muS * wS * (v[s + sK] - v[s - sK]));
-                double vz = p12zz * ((muU * uU + muP * uP) * (uU - uP) - (muD * uD + muP * uP) * (uP - uD))
+                R vz = p12zz * ((muU * uU + muP * uP) * (uU - uP) - (muD * uD + muP * uP) * (uP - uD))
                           + p12zz * ((muU * vU + muP * vP) * (vU - vP) - (muD * vD + muP * vP) * (vP - vD))
                           + p23zz * ((muU * wU + muP * wP) * (wU - wP) - (muD * wD + muP * wP) * (wP - wD))
                           + eB * (muU * wU * (u[up + sI] - u[up - sI]) - muD * wD * (u[d + sI] - u[d - sI]))
                           + eE * (muU * wU * (v[up + sJ] - v[up - sJ]) - muD * wD * (v[d + sJ] - v[d - sJ]))
                           + eD * (muU * uU * (w[up + sI] - w[up - sI]) - muD * uD * (w[d + sI] - w[d - sI]))
                           + eF * (muU * vU * (w[up + sJ] - w[up - sJ]) - muD * vD * (w[d + sJ] - w[d - sJ]));
-                double heat = p12xx * ((ka[e] + kP) * (T[e] - TP) - (ka[wq] + kP) * (TP - T[wq]))
+                R heat = p12xx * ((ka[e] + kP) * (T[e] - TP) - (ka[wq] + kP) * (TP - T[wq]))
                             + p12yy * ((ka[nn] + kP) * (T[nn] - TP) - (ka[s] + kP) * (TP - T[s]))
                             + p12zz * ((ka[up] + kP) * (T[up] - TP) - (ka[d] + kP) * (TP - T[d]));
                 K[4][n] = inv + vx + vy + vz + heat;
@@ -538,7 +577,7 @@ public:
     }
 
     // computeIntermediateSolution, src/Solver.cpp:435-442
-    void intermediate(const std::array<std::vector<double>, 5>& Ks, double inc) {
+    void intermediate(const std::array<std::vector<R>, 5>& Ks, double inc) {
         for (int q = 0; q < 5; ++q)
             for (size_t n : interior) cons[q][n] = consOld[q][n] + inc * Ks[q][n];
     }
@@ -547,7 +586,7 @@ public:
         double f = dt / 6;
         for (int q = 0; q < 5; ++q)
             for (size_t n : interior) {
-                double r = f * (k[0][q][n] + 2 * k[1][q][n] + 2 * k[2][q][n] + k[3][q][n]);
+                R r = f * (k[0][q][n] + 2 * k[1][q][n] + 2 * k[2][q][n] + k[3][q][n]);
                 cons[q][n] = consOld[q][n] + r;
             }
     }
@@ -555,11 +594,11 @@ public:
     void updatePrimitives() {
         const double gm1 = P.gamma - 1, Sc = P.sutherlands_C2 / P.T_0, ScP1 = 1 + Sc, pf = 1 / (gm1 * P.Pr);
         for (size_t n : interior) {
-            double rho = cons[0][n];
-            double u = cons[1][n] / (rho + 1), v = cons[2][n] / (rho + 1), w = cons[3][n] / (rho + 1);
-            double p = gm1 * (cons[4][n] - (rho + 1) / 2 * (u * u + v * v + w * w));
-            double T = (P.gamma * p - rho) / (1 + rho);
-            double mu = pow15(1 + T) * ScP1 / (P.Re * (T + ScP1));
+            R rho = cons[0][n];
+            R u = cons[1][n] / (rho + 1), v = cons[2][n] / (rho + 1), w = cons[3][n] / (rho + 1);
+            R p = gm1 * (cons[4][n] - (rho + 1) / 2 * (u * u + v * v + w * w));
+            R T = (P.gamma * p - rho) / (1 + rho);
+            R mu = pow15(1 + T) * ScP1 / (P.Re * (T + ScP1));
             prim[0][n] = u; prim[1][n] = v; prim[2][n] = w; prim[3][n] = p; prim[4][n] = T;
             trans[0][n] = mu; trans[1][n] = mu * pf;
         }
@@ -592,13 +631,13 @@ public:
                 rho[n] = 2 * rho[a1] - rho[a2];
             });
             // filterInletDensity, src/Boundary.cpp:142-245: in-plane neighbours inside the owned box
-            std::vector<double> filt;
+            std::vector<R> filt;
             std::vector<size_t> nodes;
             int t1 = e.axis == 0 ? 1 : 0, t2 = e.axis == 2 ? 1 : 2;  // the two in-plane axes, in loop order
             forOwned(b, [&](size_t i, size_t j, size_t kk) {
                 size_t c[3] = {i, j, kk};
                 size_t lo[3] = {b.iMin, b.jMin, b.kMin}, hi[3] = {b.iMax, b.jMax, b.kMax};
-                double acc = 0;
+                R acc = 0;
                 unsigned nb = 0;
                 for (int ax : {t1, t2}) {
                     if (c[ax] < hi[ax]) { size_t cc[3] = {c[0], c[1], c[2]}; ++cc[ax]; acc += rho[id(cc[0], cc[1], cc[2])]; ++nb; }
@@ -617,8 +656,8 @@ public:
             if (e.axis == 0) uu = vel; else if (e.axis == 1) vv = vel; else ww = vel;
             forOwned(b, [&](size_t i, size_t j, size_t kk) {
                 size_t n = id(i, j, kk);
-                double r = rho[n], T = 0;
-                double p = (T * (1 + r) + r) / P.gamma;
+                R r = rho[n], T = 0;
+                R p = (T * (1 + r) + r) / P.gamma;
                 Prim q{uu, vv, ww, p, T};
                 setNode(n, deriveCons(q), q, deriveTrans(q));
             });
@@ -626,14 +665,14 @@ public:
             forOwned(b, [&](size_t i, size_t j, size_t kk) {
                 size_t n = id(i, j, kk), a1, a2;
                 adjacent(e, n, a1, a2);
-                double r = 2 * cons[0][a1] - cons[0][a2];
-                double ru = 2 * cons[1][a1] - cons[1][a2];
-                double rv = 2 * cons[2][a1] - cons[2][a2];
-                double rw = 2 * cons[3][a1] - cons[3][a2];
-                double p = 0;
-                double u = ru / (1 + r), v = rv / (1 + r), w = rw / (1 + r);
-                double T = (P.gamma * p - r) / (1 + r);
-                double rE = p / (P.gamma - 1) + (1 + r) / 2 * (u * u + v * v + w * w);
+                R r = 2 * cons[0][a1] - cons[0][a2];
+                R ru = 2 * cons[1][a1] - cons[1][a2];
+                R rv = 2 * cons[2][a1] - cons[2][a2];
+                R rw = 2 * cons[3][a1] - cons[3][a2];
+                R p = 0;
+                R u = ru / (1 + r), v = rv / (1 + r), w = rw / (1 + r);
+                R T = (P.gamma * p - r) / (1 + r);
+                R rE = p / (P.gamma - 1) + (1 + r) / 2 * (u * u + v * v + w * w);
                 Prim q{u, v, w, p, T};
                 setNode(n, {r, ru, rv, rw, rE}, q, deriveTrans(q));
             });
@@ -649,7 +688,7 @@ public:
             forOwned(b, [&](size_t i, size_t j, size_t kk) {
                 size_t n = id(i, j, kk), a1, a2;
                 adjacent(e, n, a1, a2);
-                double u = prim[0][a2], v = prim[1][a2], w = prim[2][a2];
+                R u = prim[0][a2], v = prim[1][a2], w = prim[2][a2];
                 if (e.axis == 0) u = -u; else if (e.axis == 1) v = -v; else w = -w;
                 Prim q{u, v, w, prim[3][a2], prim[4][a2]};
                 setNode(n, deriveCons(q), q, deriveTrans(q));
@@ -682,25 +721,27 @@ public:
             }
         }
     }
-    static void lu8Solve(const double LU[64], const int rowPerm[8], const int colPerm[8], const double f[8], double x[8]) {
-        double y[8];
+    template <class S>
+    static void lu8Solve(const double LU[64], const int rowPerm[8], const int colPerm[8], const S f[8], S x[8]) {
+        S y[8];
         for (int r = 0; r < 8; ++r) y[r] = f[rowPerm[r]];
         for (int r = 1; r < 8; ++r)
             for (int c = 0; c < r; ++c) y[r] -= LU[r * 8 + c] * y[c];
-        double z[8];
+        S z[8];
         for (int r = 7; r >= 0; --r) {
-            double acc = y[r];
+            S acc = y[r];
             for (int c = r + 1; c < 8; ++c) acc -= LU[r * 8 + c] * z[c];
             z[r] = LU[r * 8 + r] != 0 ? acc / LU[r * 8 + r] : 0.0;
         }
         for (int r = 0; r < 8; ++r) x[colPerm[r]] = z[r];
     }
-    static double lu8SolveDot(const double V[64], const double f[8], V3 ip) {
+    template <class S>
+    static S lu8SolveDot(const double V[64], const S f[8], V3 ip) {
         double A[64];
         std::memcpy(A, V, sizeof(A));
         int rp[8], cp[8];
         lu8Factor(A, rp, cp);
-        double c[8];
+        S c[8];
         lu8Solve(A, rp, cp, f, c);
         // trilinearInterpolation, src/Boundary.cpp:826-833
         return c[0] + c[1] * ip.x + c[2] * ip.y + c[3] * ip.z + c[4] * ip.x * ip.y + c[5] * ip.x * ip.z + c[6] * ip.y * ip.z
@@ -711,7 +752,7 @@ public:
     void applyImmersed(const Body& B) {
         const std::vector<size_t>& filterNodes = B.filterNodes;
         // filterClosestFluidNodes (two-phase), src/Boundary.cpp:618-652
-        std::vector<double> fr, fe;
+        std::vector<R> fr, fe;
         fr.reserve(filterNodes.size());
         fe.reserve(filterNodes.size());
         const size_t sI = NJ * NK, sJ = NK;
@@ -724,7 +765,8 @@ public:
         for (size_t q = 0; q < filterNodes.size(); ++q) { cons[0][filterNodes[q]] = fr[q]; cons[4][filterNodes[q]] = fe[q]; }
         for (const Ghost& g : B.ghosts) {
             Box b = surrounding(g.ip);
-            double val[5][8], X[8], Y[8], Z[8];
+            R val[5][8];
+            double X[8], Y[8], Z[8];
             bool gflag[8] = {false, false, false, false, false, false, false, false};
             bool allFluid = true;
             V3 normals[8];
@@ -754,18 +796,18 @@ public:
                 }
                 ++c;
             }
-            double ipv[5];
+            R ipv[5];
             if (allFluid) {  // simplifiedInterpolation, src/Boundary.cpp:732-757
                 V3 lo = pos(b.iMin, b.jMin, b.kMin);
                 V3 rel = g.ip - lo;
                 for (int q = 0; q < 5; ++q) {
-                    const double* f = val[q];
-                    double a01 = f[0] + (f[1] - f[0]) * rel.z / h.z;
-                    double a23 = f[2] + (f[3] - f[2]) * rel.z / h.z;
-                    double a45 = f[4] + (f[5] - f[4]) * rel.z / h.z;
-                    double a67 = f[6] + (f[7] - f[6]) * rel.z / h.z;
-                    double a0123 = a01 + (a23 - a01) * rel.y / h.y;
-                    double a4567 = a45 + (a67 - a45) * rel.y / h.y;
+                    const R* f = val[q];
+                    R a01 = f[0] + (f[1] - f[0]) * rel.z / h.z;
+                    R a23 = f[2] + (f[3] - f[2]) * rel.z / h.z;
+                    R a45 = f[4] + (f[5] - f[4]) * rel.z / h.z;
+                    R a67 = f[6] + (f[7] - f[6]) * rel.z / h.z;
+                    R a0123 = a01 + (a23 - a01) * rel.y / h.y;
+                    R a4567 = a45 + (a67 - a45) * rel.y / h.y;
                     ipv[q] = a0123 + (a4567 - a0123) * rel.x / h.x;
                 }
             } else {  // Vandermonde, src/Boundary.cpp:788-849
@@ -832,7 +874,7 @@ public:
     // ---------------------------------------------------------------- one step, src/Solver.cpp:130-167
     void march() {
         for (int q = 0; q < 5; ++q)  // checkFinity, src/Mesh.h:48-61 (prints only; recorded here)
-            for (double x : cons[q]) if (!std::isfinite(x)) { nonFinite = true; break; }
+            for (const R& x : cons[q]) if (!std::isfinite(valueOf(x))) { nonFinite = true; break; }
         applyRhoFilter();
         std::swap(cons, consOld);  // swapConservedVariables, src/Mesh.cpp:144-151
         slopes(consOld, k[0]);
@@ -859,7 +901,7 @@ public:
     // storeNormsOfChange / getNormOfChange, src/Solver.cpp:510-530, src/Mesh.cpp:171-178
     double normOfChange(int q) const {
         double s = 0;
-        for (size_t n : interior) { double d = consOld[q][n] - cons[q][n]; s += d * d; }
+        for (size_t n : interior) { double d = valueOf(consOld[q][n]) - valueOf(cons[q][n]); s += d * d; }
         return std::sqrt(h.x * h.y * h.z * s);
     }
     void storeNorms() {
@@ -875,7 +917,7 @@ public:
         updateTimeStepSize(0);
     }
 
-    const std::vector<double>& field(int which) const {
+    const std::vector<R>& field(int which) const {
         if (which < 5) return cons[which];
         if (which < 10) return prim[which - 5];
         return trans[which - 10];
@@ -884,9 +926,9 @@ public:
     ns_config P;
     size_t NI, NJ, NK, N;
     V3 h{0, 0, 0}, off{0, 0, 0};
-    std::array<std::vector<double>, 5> cons, consOld, prim;
-    std::array<std::vector<double>, 2> trans;
-    std::array<std::array<std::vector<double>, 5>, 4> k;
+    std::array<std::vector<R>, 5> cons, consOld, prim;
+    std::array<std::vector<R>, 2> trans;
+    std::array<std::array<std::vector<R>, 5>, 4> k;
     std::vector<uint8_t> nodeType;
     std::vector<size_t> interior, solidGhost;
     std::array<std::vector<double>, 5> norms;
@@ -896,6 +938,8 @@ public:
     bool crPow = false;
     std::string err;
 };
+using Oracle = OracleT<double>;
+using TangentOracle = OracleT<Dual>;
 
 int fail(Oracle* o, const std::exception& e, int code) {
     if (o) o->err = e.what();
@@ -996,9 +1040,9 @@ int ns_oracle_derive_all(void* h) {
     auto* o = (Oracle*)h;
     if (!o) return NS_ERR_ARGUMENT;
     for (size_t n = 0; n < o->N; ++n) {
-        Cons c{o->cons[0][n], o->cons[1][n], o->cons[2][n], o->cons[3][n], o->cons[4][n]};
-        Prim q = o->derivePrim(c);
-        Trans tr = o->deriveTrans(q);
+        Oracle::Cons c{o->cons[0][n], o->cons[1][n], o->cons[2][n], o->cons[3][n], o->cons[4][n]};
+        Oracle::Prim q = o->derivePrim(c);
+        Oracle::Trans tr = o->deriveTrans(q);
         o->setNode(n, c, q, tr);
     }
     o->updateTimeStepSize(o->t);
@@ -1159,5 +1203,113 @@ void ns_oracle_lu8_solve(const double* A, const double* b, double* x) {
 
 /* Unit hook: the perturbation generator (uniform01 of the IC). */
 double ns_oracle_uniform01(uint64_t seed, uint64_t counter) { return uniform01(seed, counter); }
+
+/* ---- tangent-linear form: the same set-up and step on (value, tangent) pairs (TangentOracle). Values
+ * equal the plain oracle's bit for bit; tangents are the derivatives of the computed step with respect to
+ * the state, at the step sizes the run takes (prescribe them with ns_oracle_tl_set_dt_schedule). */
+int ns_oracle_tl_create(const ns_config* cfg, void** out) {
+    if (!cfg || !out) return NS_ERR_ARGUMENT;
+    try {
+        *out = new TangentOracle(*cfg);
+        return NS_OK;
+    } catch (const std::exception&) {
+        *out = nullptr;
+        return NS_ERR_CONFIG;
+    }
+}
+
+int ns_oracle_tl_initialize(void* h) {
+    auto* o = (TangentOracle*)h;
+    if (!o) return NS_ERR_ARGUMENT;
+    try { o->initialize(); return NS_OK; }
+    catch (const GeometryError& e) { o->err = e.what(); return NS_ERR_GEOMETRY; }
+    catch (const std::exception& e) { o->err = e.what(); return NS_ERR_CONFIG; }
+}
+
+/* Conserved state: values and tangents, 5 fields of N values each, x-fastest (as ns_oracle_get_field);
+ * primitives and transport are derived at every node from them and dt recomputed (as
+ * ns_oracle_set_field + ns_oracle_derive_all). tan may be null (zero tangents). */
+int ns_oracle_tl_set_state(void* h, const double* val, const double* tan) {
+    auto* o = (TangentOracle*)h;
+    if (!o || !val) return NS_ERR_ARGUMENT;
+    for (int q = 0; q < 5; ++q)
+        for (size_t i = 0; i < o->NI; ++i)
+            for (size_t j = 0; j < o->NJ; ++j)
+                for (size_t kk = 0; kk < o->NK; ++kk) {
+                    const size_t x = (size_t)q * o->N + (kk * o->NJ + j) * o->NI + i;
+                    o->cons[q][o->id(i, j, kk)] = Dual(val[x], tan ? tan[x] : 0.0);
+                }
+    for (size_t n = 0; n < o->N; ++n) {
+        TangentOracle::Cons c{o->cons[0][n], o->cons[1][n], o->cons[2][n], o->cons[3][n], o->cons[4][n]};
+        TangentOracle::Prim q = o->derivePrim(c);
+        o->setNode(n, c, q, o->deriveTrans(q));
+    }
+    o->updateTimeStepSize(o->t);
+    return NS_OK;
+}
+
+/* Conserved state out: values and tangents (either may be null), layout as ns_oracle_tl_set_state. */
+int ns_oracle_tl_get_state(void* h, double* val, double* tan) {
+    auto* o = (TangentOracle*)h;
+    if (!o) return NS_ERR_ARGUMENT;
+    for (int q = 0; q < 5; ++q)
+        for (size_t i = 0; i < o->NI; ++i)
+            for (size_t j = 0; j < o->NJ; ++j)
+                for (size_t kk = 0; kk < o->NK; ++kk) {
+                    const size_t x = (size_t)q * o->N + (kk * o->NJ + j) * o->NI + i;
+                    const Dual& c = o->cons[q][o->id(i, j, kk)];
+                    if (val) val[x] = c.v;
+                    if (tan) tan[x] = c.d;
+                }
+    return NS_OK;
+}
+
+int ns_oracle_tl_set_dt_schedule(void* h, const double* dts, int64_t n) {
+    auto* o = (TangentOracle*)h;
+    if (!o || n < 0) return NS_ERR_ARGUMENT;
+    o->dtSchedule.assign(dts, dts + n);
+    if (o->timeLevel < o->dtSchedule.size()) o->dt = o->dtSchedule[o->timeLevel];
+    return NS_OK;
+}
+
+int ns_oracle_tl_march(void* h, int nsteps) {
+    auto* o = (TangentOracle*)h;
+    if (!o) return NS_ERR_ARGUMENT;
+    try {
+        for (int s = 0; s < nsteps; ++s) o->march();
+        return o->nonFinite ? NS_ERR_NONFINITE : NS_OK;
+    } catch (const GeometryError& e) { o->err = e.what(); return NS_ERR_GEOMETRY; }
+    catch (const std::exception& e) { o->err = e.what(); return NS_ERR_STATE; }
+}
+
+/* RK4 slopes of the current state (as ns_oracle_slopes): values and tangents, 5 x-fastest arrays each. */
+int ns_oracle_tl_slopes(void* h, double* val, double* tan) {
+    auto* o = (TangentOracle*)h;
+    if (!o) return NS_ERR_ARGUMENT;
+    std::array<std::vector<Dual>, 5> K;
+    for (auto& a : K) a.assign(o->N, Dual());
+    o->slopes(o->cons, K);
+    for (int q = 0; q < 5; ++q)
+        for (size_t i = 0; i < o->NI; ++i)
+            for (size_t j = 0; j < o->NJ; ++j)
+                for (size_t kk = 0; kk < o->NK; ++kk) {
+                    const size_t x = (size_t)q * o->N + (kk * o->NJ + j) * o->NI + i;
+                    const Dual& c = K[q][o->id(i, j, kk)];
+                    if (val) val[x] = c.v;
+                    if (tan) tan[x] = c.d;
+                }
+    return NS_OK;
+}
+
+void ns_oracle_tl_state(void* h, double* t, uint64_t* level, double* dt) {
+    auto* o = (TangentOracle*)h;
+    if (t) *t = o->t;
+    if (level) *level = o->timeLevel;
+    if (dt) *dt = o->dt;
+}
+
+const char* ns_oracle_tl_last_error(void* h) { return h ? ((TangentOracle*)h)->err.c_str() : "null handle"; }
+
+void ns_oracle_tl_destroy(void* h) { delete (TangentOracle*)h; }
 
 }  // extern "C"

@@ -73,6 +73,19 @@ def lib():
     L.ns_oracle_set_dt_schedule.argtypes = [H, D, C.c_int64]
     L.ns_oracle_update_dt.argtypes = [H, C.c_double]
     L.ns_oracle_update_dt.restype = C.c_double
+    L.ns_oracle_tl_create.argtypes = [C.c_void_p, C.POINTER(H)]
+    L.ns_oracle_tl_initialize.argtypes = [H]
+    L.ns_oracle_tl_set_state.argtypes = [H, D, D]
+    L.ns_oracle_tl_get_state.argtypes = [H, D, D]
+    L.ns_oracle_tl_set_dt_schedule.argtypes = [H, D, C.c_int64]
+    L.ns_oracle_tl_march.argtypes = [H, C.c_int]
+    L.ns_oracle_tl_slopes.argtypes = [H, D, D]
+    L.ns_oracle_tl_state.argtypes = [H, D, C.POINTER(C.c_uint64), D]
+    L.ns_oracle_tl_state.restype = None
+    L.ns_oracle_tl_last_error.argtypes = [H]
+    L.ns_oracle_tl_last_error.restype = C.c_char_p
+    L.ns_oracle_tl_destroy.argtypes = [H]
+    L.ns_oracle_tl_destroy.restype = None
     _lib = L
     return L
 
@@ -217,6 +230,69 @@ class Oracle:
     def close(self):
         if self._h:
             lib().ns_oracle_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TangentOracle:
+    """Tangent-linear form of the oracle: the same set-up and step on (value, tangent) pairs, so each value
+    equals the plain oracle's bit for bit and each tangent is the derivative of the computed step along
+    the tangent given for the state (at the step sizes the run takes; prescribe them to hold them fixed)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self._h = C.c_void_p()
+        s = lib().ns_oracle_tl_create(C.byref(cfg), C.byref(self._h))
+        if s != 0:
+            raise OracleError(s, "create failed")
+        self.NI, self.NJ, self.NK = cfg.NI, cfg.NJ, cfg.NK
+        self.N = self.NI * self.NJ * self.NK
+
+    def _check(self, s):
+        if s != 0:
+            raise OracleError(s, lib().ns_oracle_tl_last_error(self._h).decode())
+
+    def initialize(self):
+        self._check(lib().ns_oracle_tl_initialize(self._h))
+
+    def set_state(self, values, tangents=None):
+        """Conserved state (5, NK, NJ, NI) and its tangent; primitives, transport and dt are derived."""
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        assert v.size == 5 * self.N
+        t = None if tangents is None else np.ascontiguousarray(tangents, dtype=np.float64).reshape(-1)
+        self._check(lib().ns_oracle_tl_set_state(self._h, _d(v), None if t is None else _d(t)))
+
+    def state_pair(self):
+        v, t = np.empty(5 * self.N), np.empty(5 * self.N)
+        self._check(lib().ns_oracle_tl_get_state(self._h, _d(v), _d(t)))
+        return v.reshape(5, self.NK, self.NJ, self.NI), t.reshape(5, self.NK, self.NJ, self.NI)
+
+    def set_dt_schedule(self, dts):
+        a = np.ascontiguousarray(dts, dtype=np.float64)
+        self._check(lib().ns_oracle_tl_set_dt_schedule(self._h, _d(a), a.size))
+
+    def march(self, n=1):
+        self._check(lib().ns_oracle_tl_march(self._h, int(n)))
+
+    def slopes(self):
+        v, t = np.zeros(5 * self.N), np.zeros(5 * self.N)
+        self._check(lib().ns_oracle_tl_slopes(self._h, _d(v), _d(t)))
+        return v.reshape(5, self.NK, self.NJ, self.NI), t.reshape(5, self.NK, self.NJ, self.NI)
+
+    def state(self):
+        t, dt = C.c_double(), C.c_double()
+        lv = C.c_uint64()
+        lib().ns_oracle_tl_state(self._h, C.byref(t), C.byref(lv), C.byref(dt))
+        return t.value, lv.value, dt.value
+
+    def close(self):
+        if self._h:
+            lib().ns_oracle_tl_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -44,6 +44,16 @@ double ns_oracle_norm_of_change(void* h, int q);
 double ns_oracle_update_dt(void* h, double tnow);
 void ns_oracle_lu8_solve(const double* A, const double* b, double* x);
 double ns_oracle_uniform01(uint64_t seed, uint64_t counter);
+int ns_oracle_tl_create(const ns_config* cfg, void** out);
+int ns_oracle_tl_initialize(void* h);
+int ns_oracle_tl_set_state(void* h, const double* val, const double* tan);
+int ns_oracle_tl_get_state(void* h, double* val, double* tan);
+int ns_oracle_tl_set_dt_schedule(void* h, const double* dts, int64_t n);
+int ns_oracle_tl_march(void* h, int nsteps);
+int ns_oracle_tl_slopes(void* h, double* val, double* tan);
+void ns_oracle_tl_state(void* h, double* t, uint64_t* level, double* dt);
+const char* ns_oracle_tl_last_error(void* h);
+void ns_oracle_tl_destroy(void* h);
 }
 
 static int failures = 0;
@@ -113,6 +123,25 @@ static void exercise(const char* name, const ns_config& c, int steps) {
     ns_oracle_state(h, &t, &lv, &dt);
     ns_oracle_geometry(h, sp, of);
     ns_oracle_destroy(h);
+    // the tangent-linear form: same set-up, a tangent on every state value, two steps at the first dt
+    void* g = nullptr;
+    CHECK(ns_oracle_tl_create(&c, &g) == NS_OK);
+    if (g) {
+        CHECK(ns_oracle_tl_initialize(g) == NS_OK);
+        std::vector<double> v(5 * N), d(5 * N);
+        CHECK(ns_oracle_tl_get_state(g, v.data(), d.data()) == NS_OK);
+        for (size_t x = 0; x < d.size(); ++x) d[x] = (double)((x * 2654435761u) % 1000) * 1e-3 - 0.5;
+        CHECK(ns_oracle_tl_set_state(g, v.data(), d.data()) == NS_OK);
+        double tt, dd;
+        uint64_t ll;
+        ns_oracle_tl_state(g, &tt, &ll, &dd);
+        const double dts[2] = {dd, dd};
+        CHECK(ns_oracle_tl_set_dt_schedule(g, dts, 2) == NS_OK);
+        CHECK(ns_oracle_tl_march(g, 2) == NS_OK);
+        CHECK(ns_oracle_tl_slopes(g, v.data(), d.data()) == NS_OK);
+        CHECK(ns_oracle_tl_last_error(g) != nullptr);
+        ns_oracle_tl_destroy(g);
+    }
     std::printf("%-28s ghosts %6lld filter %6lld interior %8lld  dt %.6e\n", name, (long long)ng, (long long)nf,
                 (long long)ni, dt);
     (void)t;
