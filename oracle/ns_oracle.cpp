@@ -183,7 +183,7 @@ public:
     }
     Trans deriveTrans(const Prim& q) const {
         double ScPlusOne = 1 + P.sutherlands_C2 / P.T_0;
-        R mu = pow15(1 + q.T) * ScPlusOne / (P.Re * (q.T + ScPlusOne));
+        R mu = pow15(1 + q.T) * ScPlusOne / (Re * (q.T + ScPlusOne));
         R kappa = mu / ((P.gamma - 1) * P.Pr);
         return {mu, kappa};
     }
@@ -598,7 +598,7 @@ public:
             R u = cons[1][n] / (rho + 1), v = cons[2][n] / (rho + 1), w = cons[3][n] / (rho + 1);
             R p = gm1 * (cons[4][n] - (rho + 1) / 2 * (u * u + v * v + w * w));
             R T = (P.gamma * p - rho) / (1 + rho);
-            R mu = pow15(1 + T) * ScP1 / (P.Re * (T + ScP1));
+            R mu = pow15(1 + T) * ScP1 / (Re * (T + ScP1));
             prim[0][n] = u; prim[1][n] = v; prim[2][n] = w; prim[3][n] = p; prim[4][n] = T;
             trans[0][n] = mu; trans[1][n] = mu * pf;
         }
@@ -924,6 +924,7 @@ public:
     }
 
     ns_config P;
+    R Re = P.Re;  // the Reynolds number of the viscosity law (a tangent may be set on it)
     size_t NI, NJ, NK, N;
     V3 h{0, 0, 0}, off{0, 0, 0};
     std::array<std::vector<R>, 5> cons, consOld, prim;
@@ -1261,6 +1262,15 @@ int ns_oracle_tl_get_state(void* h, double* val, double* tan) {
                     if (val) val[x] = c.v;
                     if (tan) tan[x] = c.d;
                 }
+    return NS_OK;
+}
+
+/* Tangent of the Reynolds number of the viscosity law (src/Solver.cpp:476): with zero state tangents,
+ * the tangents of a run are then the derivatives with respect to Re. Set before ns_oracle_tl_set_state. */
+int ns_oracle_tl_set_re_tangent(void* h, double dRe) {
+    auto* o = (TangentOracle*)h;
+    if (!o) return NS_ERR_ARGUMENT;
+    o->Re = Dual(o->P.Re, dRe);
     return NS_OK;
 }
 

@@ -78,6 +78,7 @@ def lib():
     L.ns_oracle_tl_set_state.argtypes = [H, D, D]
     L.ns_oracle_tl_get_state.argtypes = [H, D, D]
     L.ns_oracle_tl_set_dt_schedule.argtypes = [H, D, C.c_int64]
+    L.ns_oracle_tl_set_re_tangent.argtypes = [H, C.c_double]
     L.ns_oracle_tl_march.argtypes = [H, C.c_int]
     L.ns_oracle_tl_slopes.argtypes = [H, D, D]
     L.ns_oracle_tl_state.argtypes = [H, D, C.POINTER(C.c_uint64), D]
@@ -275,6 +276,10 @@ class TangentOracle:
     def set_dt_schedule(self, dts):
         a = np.ascontiguousarray(dts, dtype=np.float64)
         self._check(lib().ns_oracle_tl_set_dt_schedule(self._h, _d(a), a.size))
+
+    def set_re_tangent(self, dre):
+        """Tangent of the Reynolds number (before set_state, which derives the viscosity from it)."""
+        self._check(lib().ns_oracle_tl_set_re_tangent(self._h, float(dre)))
 
     def march(self, n=1):
         self._check(lib().ns_oracle_tl_march(self._h, int(n)))

@@ -235,7 +235,8 @@ def test_gradient_against_the_tangent_linear_oracle():
     the mean over levels n0..n1 of minus the node volume times the sum of the tangents of the oracle's
     x-momentum slopes, from one tangent run started at (U0, d) at the device run's step sizes. The values of
     that run give J itself (the plain oracle's, bit for bit); the derivatives equal the adjoint's <grad, d>
-    to 1e-10 on three random directions (measured 1.4e-13 to 8.2e-13), with no difference step involved."""
+    to 1e-10 on three random directions (measured 1.4e-13 to 8.2e-13), with no difference step involved;
+    a tangent on the Reynolds number alone gives dJ/dRe against the adjoint's."""
     import oracle
 
     cfg, U, r, (nsteps, n0, n1) = periodic_sphere_adjoint()
@@ -268,4 +269,20 @@ def test_gradient_against_the_tangent_linear_oracle():
         rel = abs(dJ - ad) / abs(dJ)
         print(f"  tangent-linear oracle dJ={dJ:.12e} adjoint={ad:.12e} rel={rel:.2e}", flush=True)
         worst = max(worst, rel)
+    # the Reynolds number: a tangent on Re alone
+    tl = oracle.TangentOracle(cfg)
+    tl.initialize()
+    tl.set_re_tangent(1.0)
+    tl.set_state(U)
+    tl.set_dt_schedule(r["dts"])
+    dcd = []
+    for level in range(1, nsteps + 1):
+        tl.march(1)
+        if level >= n0:
+            dcd.append(-vol * tl.slopes()[1][1].sum() / scale)
+    tl.close()
+    dJ = float(np.mean(dcd))
+    rel = abs(dJ - r["grad_re"]) / abs(dJ)
+    print(f"  tangent-linear oracle dJ/dRe={dJ:.12e} adjoint={r['grad_re']:.12e} rel={rel:.2e}", flush=True)
+    worst = max(worst, rel)
     assert worst <= 1e-10, worst

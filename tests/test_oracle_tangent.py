@@ -100,3 +100,29 @@ def test_tangents_match_central_differences_of_the_oracle(name):
             for q in range(5):
                 rel = np.linalg.norm(tan[q] - fd[q]) / np.linalg.norm(tan[q])
                 assert rel <= 5e-8, (name, q, rel)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_reynolds_tangent_matches_central_differences(name):
+    """A tangent on Re alone (zero state tangent): the run's tangents are the derivatives with respect to
+    the Reynolds number of the viscosity law, against central differences of the plain oracle in Re."""
+    cfg, U0, dts = setup(name)
+    tl = oracle.TangentOracle(cfg)
+    tl.initialize()
+    tl.set_re_tangent(1.0)
+    tl.set_state(U0)
+    tl.set_dt_schedule(dts)
+    tl.march(STEPS)
+    _, dU = tl.state_pair()
+    _, dK = tl.slopes()
+    tl.close()
+    h = 1e-4 * cfg.Re  # measured: <= 6e-8 per field here, the differences' own rounding (it grows as h shrinks)
+    runs = []
+    for re in (cfg.Re + h, cfg.Re - h):
+        c = CASES[name]()
+        c.Re = re
+        runs.append(plain_run(c, U0, dts, STEPS))
+    for tan, fd in ((dU, (runs[0][0] - runs[1][0]) / (2 * h)), (dK, (runs[0][1] - runs[1][1]) / (2 * h))):
+        for q in range(5):
+            rel = np.linalg.norm(tan[q] - fd[q]) / np.linalg.norm(tan[q])
+            assert rel <= 2e-7, (name, q, rel)
