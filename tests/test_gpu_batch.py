@@ -145,3 +145,72 @@ def test_batch_with_periodic_density_filter(graph):
     nofilter.kahan = 1
     U0, _, _ = single(nofilter, steps, tile)
     assert not np.array_equal(U0, filtered)
+
+
+def test_full_sweep_of_64_cases_500_steps():
+    """The sweep as specified: 64 seeded cases (grid edges 32-128), 500 steps each at its own dt, in one
+    batch. Four cases of this draw (31: 32^3, Mach 0.38, sphere radius 0.53; 52: 48^3, r 0.31; 39 and 42:
+    64^3, r 0.32-0.34, Re 1700-3800) go non-finite within the 500 steps: the serial FP64 oracle does too,
+    at steps 70, 205, 243 and 322 (the reference's non-dissipative central scheme on spheres of 4-5 cells
+    radius; its periodic density filter does not change this). As the reference's checkFinity
+    (src/Mesh.h:48-61) reports and goes on, the batch finishes the steps of every case and names the first
+    such case; every other case stays finite, and three of them (64^3, 96^3, 128^3) equal their single-case
+    runs bit for bit. Case 31 in the device's FP64 form first holds a non-finite value at the oracle's
+    step."""
+    import oracle
+
+    steps = 500
+    cfgs = cases.sweep64(n=64, steps=steps)
+    b = nsfsi.Batch(cfgs)
+    b.initialize()
+    tile = b.tile()
+    print("sweep64 batch, 500 steps, tile", tile, flush=True)
+    with pytest.raises(nsfsi.NSError) as e:
+        b.step(steps)
+    assert e.value.status == 3 and "of case 31" in str(e.value), str(e.value)
+    bad = [c for c in range(64) if not np.all(np.isfinite(b.conserved(c)))]
+    assert bad == [31, 39, 42, 52], bad
+    for c in range(64):
+        assert b.time(c)[1] == steps
+    picked = {}
+    for c, cfg in enumerate(cfgs):
+        if c not in bad and cfg.NI in (64, 96, 128):
+            picked.setdefault(cfg.NI, c)
+    for N, c in sorted(picked.items()):
+        print(f"case {c} ({N}^3) against its single run", flush=True)
+        U, (t, lv, dt), _ = single(cfgs[c], steps, tile)
+        assert b.time(c) == (t, lv, dt)
+        assert np.array_equal(b.conserved(c), U), c
+    b.close()
+
+    def first_nonfinite(step, finite, limit):
+        for n in range(1, limit + 1):
+            step()
+            if not finite():
+                return n
+        return None
+
+    c31 = cases.sweep64(n=64, steps=steps, precision="fp64")[31]
+    o = oracle.Oracle(c31)
+    o.initialize()
+    def oracle_step():
+        try:
+            o.march(1)
+        except oracle.OracleError:  # reports the non-finite state of the step before and goes on
+            pass
+
+    n_oracle = first_nonfinite(oracle_step, lambda: np.all(np.isfinite(o.field(0))), 80)
+    o.close()
+    sv = nsfsi.Solver(c31)
+    sv.initialize()
+
+    def dev_step():
+        try:
+            sv.marchTimeStep(1)
+        except nsfsi.NSError:
+            pass
+
+    n_dev = first_nonfinite(dev_step, lambda: np.all(np.isfinite(sv.mesh.field("rho"))), 80)
+    sv.close()
+    print(f"case 31 FP64: first non-finite step oracle {n_oracle}, device {n_dev}", flush=True)
+    assert n_oracle == n_dev == 70
