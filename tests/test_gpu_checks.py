@@ -170,15 +170,18 @@ def test_sphere512_geometry_fp64_against_oracle_at_96():
 
 
 def test_sphere512_full_size_fp32_against_fp64():
-    """Full 512^3: 50 steps of the FP32 compensated form against the FP64 form on the device (which matches
-    the oracle per the tests above): relative L2 <= 1e-5 per conserved field, and the same step count."""
-    steps = 50
+    """Full 512^3 for the specified 1000 steps: the FP32 compensated form against the FP64 form on the device
+    (which matches the oracle per the tests above): relative L2 <= 1e-5 per conserved field (measured
+    <= 1.4e-6), and the same step count."""
+    steps = 1000
     out = {}
     for prec in ("fp64", "fp32"):
         cfg = cases.sphere512(N=512, steps=steps, precision=prec)
         cfg.kahan = 1
         s = solver(cfg, graph=1)
-        s.marchTimeStep(steps)
+        for done in range(0, steps, 250):
+            s.marchTimeStep(250)
+            print(f"  {prec}: {done + 250} steps", flush=True)
         out[prec] = (s.mesh.conserved(), s.time())
         s.close()
     U64, (t64, l64, _) = out["fp64"]
