@@ -229,7 +229,8 @@ def test_cylinder256_1000_steps_fp32_against_fp64():
 def test_adjoint_sphere96_against_central_differences():
     """96^3, 400 FP64 steps, J = mean C_D over levels 300..400, checkpoint every 20 steps: the adjoint
     directional derivative agrees with central differences (eps = 1e-4, same frozen step sizes) to 1e-4
-    relative (measured about 1e-8) on three random initial-state directions and on Re."""
+    relative on the specified 16 random initial-state directions (measured <= 7.1e-7; the directions whose
+    derivative is small lose most to the differences' rounding) and on Re (7e-8)."""
     cfg = cases.sphere96(N=96, steps=400)
     s = solver(cfg)
     U = s.mesh.conserved()
@@ -243,7 +244,7 @@ def test_adjoint_sphere96_against_central_differences():
 
     rng = np.random.default_rng(96)
     worst = 0.0
-    for d in range(3):
+    for d in range(16):
         dU = rng.standard_normal(U.shape)
         dU /= np.abs(dU).max()
         fd = (J(U + eps * dU, cfg.Re) - J(U - eps * dU, cfg.Re)) / (2 * eps)
