@@ -389,7 +389,7 @@ __global__ void k_adj_filter_give(Layout L, Window W, const int64_t* fn, int64_t
     const size_t w = W.idx(i, j, k);
     atomAdd(&Sb[o], 6.0 * br + Pb[w]);
     atomAdd(&Sb[o + E4], 6.0 * be + PEb[w]);
-    const long nbs[6] = {L.xstep(i, 1), L.xstep(i, -1), sy, -sy, sz, -sz};
+    const long nbs[6] = {Layout::xstep(i, 1), Layout::xstep(i, -1), sy, -sy, sz, -sz};
     for (int r = 0; r < 6; ++r) {
         atomAdd(&Sb[o + nbs[r]], br);
         atomAdd(&Sb[o + nbs[r] + E4], be);
@@ -412,8 +412,8 @@ __global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const dou
     // offsets of the nodes one and two in from the face (x-steps depend on the column: padding slots)
     const int dn = f.side == 0 ? 1 : -1;
     const long st = f.axis == 1 ? (long)L.pitch : (long)L.plane;
-    const long s1 = f.axis == 0 ? L.xstep(c[0], dn) : dn * st;
-    const long s2 = f.axis == 0 ? L.xstep(c[0], 2 * dn) : 2 * dn * st;
+    const long s1 = f.axis == 0 ? Layout::xstep(c[0], dn) : dn * st;
+    const long s2 = f.axis == 0 ? Layout::xstep(c[0], 2 * dn) : 2 * dn * st;
     double ob[5];
     for (int r = 0; r < 5; ++r) {
         ob[r] = Sb[o + r * F];
@@ -463,8 +463,8 @@ __global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const dou
         unsigned nbn = 0;
         long nbo[4];
         for (int ax : {ax1, ax2}) {
-            const long up = ax == 0 ? L.xstep(c[0], 1) : (ax == 1 ? (long)L.pitch : (long)L.plane);
-            const long down = ax == 0 ? L.xstep(c[0], -1) : -up;
+            const long up = ax == 0 ? Layout::xstep(c[0], 1) : (ax == 1 ? (long)L.pitch : (long)L.plane);
+            const long down = ax == 0 ? Layout::xstep(c[0], -1) : -up;
             if (c[ax] < hi[ax]) nbo[nbn++] = up;
             if (c[ax] > lo[ax]) nbo[nbn++] = down;
         }

@@ -28,8 +28,7 @@ Batch::Batch(const ns_config* cfgs, int n) {
         if (k.precision != NS_FP32) throw Unsupported("batched cases run in FP32 (precision = fp32)");
         if (k.kahan != cfgs[0].kahan) throw Unsupported("all batched cases need the same kahan setting");
         if (k.n_extra_bodies != 0) throw Unsupported("batched cases hold at most one immersed body");
-        // 62-node segments: the batched step runs one stage per launch, and small rows fill more lanes
-        cases.push_back(std::make_unique<Case>(k, 62));
+        cases.push_back(std::make_unique<Case>(k));
     }
     kahan = cfgs[0].kahan != 0;
     hipCheck(hipStreamCreate(&stream), "stream");

@@ -26,9 +26,7 @@ void hipCheck(hipError_t e, const char* what);
 
 class Case {
 public:
-    // segNodes: nodes per x-segment of the layout (62, or 60 in FP32); 0 = 60 for FP32 cases whose steps can
-    // run as paired stages (no periodic density filter), else 62
-    explicit Case(const ns_config& c, int segNodes = 0);
+    explicit Case(const ns_config& c);
     ~Case();
     Case(const Case&) = delete;
     Case& operator=(const Case&) = delete;

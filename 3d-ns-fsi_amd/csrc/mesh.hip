@@ -368,16 +368,16 @@ __device__ __forceinline__ void edgeFaceNode(const Layout& L, const Params& P, c
     // offsets of the nodes one and two in from the face (x-steps depend on the column: padding slots)
     const int dn = f.side == 0 ? 1 : -1;  // towards the interior
     const long st = f.axis == 1 ? (long)L.pitch : (long)L.plane;
-    const long s1 = f.axis == 0 ? L.xstep(c[0], dn) : dn * st;
-    const long s2 = f.axis == 0 ? L.xstep(c[0], 2 * dn) : 2 * dn * st;
+    const long s1 = f.axis == 0 ? Layout::xstep(c[0], dn) : dn * st;
+    const long s2 = f.axis == 0 ? Layout::xstep(c[0], 2 * dn) : 2 * dn * st;
     if (f.type == NS_BC_INLET) {
         auto extrap = [&](size_t oo) { return (T)2 * F.f[0][oo + s1] - F.f[0][oo + s2]; };
         // filterInletDensity, src/Boundary.cpp:142-245: neighbours inside the owned box, order +a1,-a1,+a2,-a2
         T acc = 0;
         unsigned nb = 0;
         for (int ax : {ax1, ax2}) {
-            const long up = ax == 0 ? L.xstep(c[0], 1) : (ax == 1 ? (long)L.pitch : (long)L.plane);
-            const long down = ax == 0 ? L.xstep(c[0], -1) : -up;
+            const long up = ax == 0 ? Layout::xstep(c[0], 1) : (ax == 1 ? (long)L.pitch : (long)L.plane);
+            const long down = ax == 0 ? Layout::xstep(c[0], -1) : -up;
             if (c[ax] < hi[ax]) { acc += extrap(o + up); ++nb; }
             if (c[ax] > lo[ax]) { acc += extrap(o + down); ++nb; }
         }
@@ -483,8 +483,8 @@ __global__ void k_edge_faces(Layout L, Params P, EdgeFaceGroup g, T* S, const do
     if (g.rowSlots[q]) {
         if (a >= L.pitch) return;
         const int r = a % Layout::kSegSlots;
-        const int i = (a / Layout::kSegSlots) * L.sn + r - L.sp;
-        if (r < L.sp || r >= L.sp + L.sn || i >= L.NI) {
+        const int i = (a / Layout::kSegSlots) * Layout::kSegNodes + r - 1;
+        if (r == 0 || r == Layout::kSegSlots - 1 || i >= L.NI) {
             const int j = f.axis == 1 ? f.j0 : f.j0 + bb, k = f.axis == 1 ? f.k0 + bb : f.k0;
             const size_t o = ((size_t)k * L.NJ + j) * L.pitch + a;
             for (int c = 0; c < 5; ++c) S[o + c * L.fstride] = (T)0;
@@ -530,7 +530,7 @@ __device__ __forceinline__ void filterComputeNode(const Layout& L, const Window&
     size_t o = L.off(i, j, k);
     const T* r = S;
     const T* E = S + 4 * L.fstride;
-    const long xp = L.xstep(i, 1), xm = L.xstep(i, -1), sy = L.pitch, sz = (long)L.plane;
+    const long xp = Layout::xstep(i, 1), xm = Layout::xstep(i, -1), sy = L.pitch, sz = (long)L.plane;
     // order of the reference sum: P*6, i+1, i-1, j+1, j-1, k+1, k-1
     T fr = ((T)6 * r[o] + r[o + xp] + r[o + xm] + r[o + sy] + r[o - sy] + r[o + sz] + r[o - sz]) / (T)12;
     T fe = ((T)6 * E[o] + E[o + xp] + E[o + xm] + E[o + sy] + E[o - sy] + E[o + sz] + E[o - sz]) / (T)12;

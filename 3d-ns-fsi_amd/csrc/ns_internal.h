@@ -2,10 +2,8 @@
  * ns_internal.h — shared host/device declarations of the MI355X Navier–Stokes step.
  *
  * Device field layout: structure of arrays, x fastest, x-rows in segments. The nodes of a row are cut into
- * segments of sn consecutive nodes (62, or 60 for the paired-stage FP32 form, whose two chained stencils
- * need two halo lanes on each side); segment s occupies the 64 slots 64 s .. 64 s + 63 of the row and node
- * i = sn s + r sits in slot 64 s + r + sp, sp = (64 - sn) / 2, so the first and last sp slots of every
- * segment hold no node (padding). A
+ * segments of 62 consecutive nodes; segment s occupies the 64 slots 64 s .. 64 s + 63 of the row and node
+ * i = 62 s + r sits in slot 64 s + r + 1, so slots 0 and 63 of every segment hold no node (padding). A
  * segment is then 256 B (FP32) or 512 B (FP64) and starts on such a boundary: the fused stage kernel,
  * whose x-tiles are the segments, writes whole aligned cache lines (a line written in part costs a
  * read-modify-write in the memory system). Every x-row is padded (pitch) to whole segments in FP32 (so the
@@ -43,23 +41,21 @@ enum : uint8_t {
 };
 
 struct Layout {
-    static constexpr int kSegSlots = 64;  // slots per x-segment (the first and last sp are padding)
+    static constexpr int kSegNodes = 62;  // nodes per x-segment
+    static constexpr int kSegSlots = 64;  // slots per x-segment (first and last are padding)
     int NI, NJ, NK;
-    int sn;           // nodes per x-segment: 62 or 60
-    int sp;           // padding slots before a segment's first node: (64 - sn) / 2
     int pitch;        // slots per padded x-row
     size_t plane;     // NJ * pitch
     size_t fstride;   // elements between consecutive fields (>= NK * plane, 256-B aligned)
     NS_HD size_t nodes() const { return (size_t)NI * NJ * NK; }
-    // slot of node i in its row (i >= -sp; nodes -sp .. -1 map to the padding slots of segment 0)
-    NS_HD int col(int i) const { return i + (kSegSlots - sn) * (i / sn) + sp; }
+    // slot of node i in its row (i >= -1; node -1 maps to the padding slot 0)
+    NS_HD static int col(int i) { return i + 2 * (i / kSegNodes) + 1; }
     NS_HD size_t off(int i, int j, int k) const { return ((size_t)k * NJ + j) * pitch + col(i); }
     // off(i + d, j, k) - off(i, j, k): the x-step of d nodes from node i (crosses padding slots)
-    NS_HD long xstep(int i, int d) const { return (long)col(i + d) - col(i); }
+    NS_HD static long xstep(int i, int d) { return (long)col(i + d) - col(i); }
 };
 
-// segNodes: 62, or 60 (FP32 only: the paired-stage form)
-Layout makeLayout(int NI, int NJ, int NK, size_t elemBytes, int segNodes = 62);
+Layout makeLayout(int NI, int NJ, int NK, size_t elemBytes);
 
 // Axis-aligned index window holding every solid, ghost and filter node of the body.
 struct Window {

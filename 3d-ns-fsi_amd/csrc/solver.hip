@@ -33,18 +33,15 @@ void hipCheck(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-Layout makeLayout(int NI, int NJ, int NK, size_t eb, int segNodes) {
-    if (segNodes != 62 && !(segNodes == 60 && eb == 4)) throw std::invalid_argument("segment length 62, or 60 in FP32");
+Layout makeLayout(int NI, int NJ, int NK, size_t eb) {
     Layout L;
     L.NI = NI;
     L.NJ = NJ;
     L.NK = NK;
-    L.sn = segNodes;
-    L.sp = (Layout::kSegSlots - segNodes) / 2;
     // FP32 rows hold whole segments (256 B), so the 8-bit compensation carry, stored at a quarter of the byte
     // offsets, covers 64 aligned bytes per segment; FP64 rows are padded to 128 B
     const size_t rowElems = eb == 4 ? (size_t)Layout::kSegSlots : 128 / eb;
-    const size_t slots = (size_t)L.col(NI - 1) + 1;  // through the last node's slot
+    const size_t slots = (size_t)Layout::col(NI - 1) + 1;  // through the last node's slot
     L.pitch = (int)((slots + rowElems - 1) / rowElems * rowElems);
     L.plane = (size_t)NJ * L.pitch;
     size_t fe = 256 / eb;
@@ -135,7 +132,7 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ------------------------------------------------------------------------------------------------ Case
 
-Case::Case(const ns_config& c, int segNodes) : cfg(c) {
+Case::Case(const ns_config& c) : cfg(c) {
     setDerived(cfg);
     std::string v = validateConfig(cfg);
     if (!v.empty()) throw ConfigError(v);
@@ -150,9 +147,7 @@ Case::Case(const ns_config& c, int segNodes) : cfg(c) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         throw HipError(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
     hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-    // the paired-stage form chains two stencils in one pass, so its x-tiles need two halo lanes on each side
-    if (!segNodes) segNodes = fp32 && filterDisabled() ? 60 : 62;
-    L = makeLayout(cfg.NI, cfg.NJ, cfg.NK, eb, segNodes);
+    L = makeLayout(cfg.NI, cfg.NJ, cfg.NK, eb);
 }
 
 Case::~Case() {

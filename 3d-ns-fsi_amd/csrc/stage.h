@@ -85,7 +85,7 @@ int launchStageFused(hipStream_t s, int stage, const Layout& L, const Params& P,
 // The caller also keeps f out of the edge-face pass and checks that f is independent of every face
 // applied before it.
 inline bool outletFusable(const Layout& L, const EdgeFace& f) {
-    const int tail = L.NI - L.sn * ((L.NI + L.sn - 1) / L.sn - 1);
+    const int tail = L.NI - Layout::kSegNodes * ((L.NI + Layout::kSegNodes - 1) / Layout::kSegNodes - 1);
     return f.type == NS_BC_OUTLET && f.axis == 0 && f.side == 1 && f.i0 == L.NI - 1 && f.i1 == L.NI - 1 && tail >= 3;
 }
 const int* fusedTileHeights(int* n);

@@ -13,10 +13,9 @@
  * is computed once and shared by the two nodes of the face.
  *
  * Tiling. A block is 64 x (TY + 2) threads: one wavefront per x-row. The x-tiles are the layout's
- * x-segments (sn = 62 or 60 nodes in 64 slots, sp = 1 or 2 padding slots on each side): lanes sp..sp+sn-1
- * own the segment's nodes, lanes sp-1 and sp+sn are the x-halo (they read the neighbouring segments' edge
- * nodes; with sn = 60 lanes 0 and 63 are idle); rows 1..TY own output rows, rows 0 and TY+1 the
- * y-halo. A row's last segment, when it holds at most 32 - 2 sp nodes, is tiled at half width: each half of the
+ * x-segments (62 nodes in 64 slots): lanes 1..62 own the segment's nodes, lanes 0 and 63 are the x-halo
+ * (they read the neighbouring segments' edge nodes); rows 1..TY own output rows, rows 0 and TY+1 the
+ * y-halo. A row's last segment, when it holds at most 30 nodes, is tiled at half width: each half of the
  * wave takes it in one of two y-bands, so a 512-node row costs 8.5 waves instead of 9 and a 64-node row
  * 1.5 instead of 2. The block streams through a chunk of z-planes: the own column's neighbours in z live in
  * registers, x-neighbours come by cross-lane shuffle, y-neighbours through LDS (two buffers in FP32, so a
@@ -91,7 +90,7 @@ struct FusedArgs {
     const T* rhoEPre;
     const double* dtdev;
     int nbx, nby, nbz, kc;   // tiles in x, y, z and planes per z-chunk
-    int tailHalf;            // the last x-segment holds at most 32 - 2 sp nodes: its tiles are half-width
+    int tailHalf;            // the last x-segment holds at most 30 nodes: its tiles are half-width
     int nblocks;
     // outlet face at x = NI-1 applied here (its owned rows j0..j1 x k0..k1), instead of by the edge-face pass
     int fuseOutlet, oj0, oj1, ok0, ok1;
@@ -453,8 +452,8 @@ __device__ __forceinline__ TileIdx tileOf(int lb, int nbx, int nby, bool tailHal
 }
 
 // One tile (block lb of the case described by a): the whole stage for one x-segment x TY output rows over one
-// z-chunk. When a row's last segment holds at most 32 - 2 sp nodes, its tiles are half-width: lanes 0..31 take the
-// segment in y-band by, lanes 32..63 in band by+1, each half with its own halo lanes (sub-lane sp-1 and the
+// z-chunk. When a row's last segment holds at most 30 nodes, its tiles are half-width: lanes 0..31 take the
+// segment in y-band by, lanes 32..63 in band by+1, each half with its own halo lanes (sub-lane 0 and the
 // sub-lanes past the last node), so a wave shift across the halves only moves values between halo lanes.
 template <class T, int STAGE, int TY, bool SCAN, bool KAHAN, bool BUF>
 __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, Smem<T, TY> (&smb)[smemBuffers<T>()],
@@ -466,11 +465,11 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     const int sl = tl.half ? lane & 31 : lane;            // lane within the (half-)tile
     const int by = tl.half ? tl.by + (lane >> 5) : tl.by;  // y-band of this lane
     const Layout& L = a.L;
-    const int i = bx * L.sn + sl - L.sp;
+    const int i = bx * 62 + sl - 1;
     const int j = by * TY + row - 1;
     const int kb = bz * a.kc, ke = min(L.NK, kb + a.kc);
     const bool outRow = row >= 1 && row <= TY && j < L.NJ;
-    const bool outLane = sl >= L.sp && sl < L.sp + L.sn && i < L.NI;
+    const bool outLane = sl >= 1 && sl <= 62 && i < L.NI;
     const Coef<T>& c = a.c;
     const double dtd = *a.dtdev;
     const bool rhoFiltered = STAGE == 1 && a.fFlag && *a.fFlag;
@@ -480,8 +479,8 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     // clamped column of the stencil point (halo lanes and rows beyond the grid repeat the edge); the x-tile
     // is segment bx of the layout, so the halo lanes read the neighbouring segments
     const int ic = min(max(i, 0), L.NI - 1), jc = min(max(j, 0), L.NJ - 1);
-    const uint32_t colc = slotBytes<T>(L, L.col(ic), jc);
-    // the own slot: sub-lane l writes slot l of segment bx (the first and last sp sub-lanes the padding slots)
+    const uint32_t colc = slotBytes<T>(L, Layout::col(ic), jc);
+    // the own slot: sub-lane l writes slot l of segment bx (sub-lanes 0 and 63 the padding slots)
     const uint32_t colo = slotBytes<T>(L, Layout::kSegSlots * bx + sl, j);
     const uint32_t planeB = (uint32_t)(L.plane * sizeof(T));
     // Slots without a node that this wave writes (values never read): the segment's padding slots and, in
@@ -490,8 +489,8 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     // half-width tile writes the carry's upper half with a second store), so that no line (carry: no
     // aligned 64-byte half) of the row is written in part. (FP64 ran 0.5 % slower with its last line completed on 512-node rows, whose last line holds
     // one node and fifteen slots past it.)
-    const int slot = Layout::kSegSlots * bx + sl, endSlot = L.col(L.NI - 1) + 1;
-    const bool padLane = sl < L.sp || sl >= L.sp + L.sn || i >= L.NI;
+    const int slot = Layout::kSegSlots * bx + sl, endSlot = Layout::col(L.NI - 1) + 1;
+    const bool padLane = sl == 0 || sl == 63 || i >= L.NI;
     const bool padLine = padLane && slot < (sizeof(T) == 4 ? (endSlot + 31) / 32 * 32 : endSlot);
     const int segEnd = min(L.pitch, (endSlot + 63) / 64 * 64);
     const bool padSeg = padLane && slot < segEnd;
@@ -930,7 +929,7 @@ __global__ void k_force_finalize(const double* partials, int nb, double vol, Ste
 
 template <class T, int TY>
 int fusedBlocks(const Layout& L, int kc) {
-    return ((L.NI + L.sn - 1) / L.sn) * ((L.NJ + TY - 1) / TY) * ((L.NK + kc - 1) / kc);
+    return ((L.NI + 61) / 62) * ((L.NJ + TY - 1) / TY) * ((L.NK + kc - 1) / kc);
 }
 
 template <class T, int TY, bool K, bool B>
@@ -1010,10 +1009,10 @@ void fillArgs(FusedArgs<T>& a, int stage, const Layout& L, const Params& P, cons
     a.dtdev = dtdev;
     const int ty = normTY(tile.ty);
     a.kc = chunkPlanes(tile);
-    a.nbx = (L.NI + L.sn - 1) / L.sn;
+    a.nbx = (L.NI + 61) / 62;
     a.nby = (L.NJ + ty - 1) / ty;
     a.nbz = (L.NK + a.kc - 1) / a.kc;
-    a.tailHalf = L.NI - L.sn * (a.nbx - 1) <= 32 - 2 * L.sp;
+    a.tailHalf = L.NI - Layout::kSegNodes * (a.nbx - 1) <= 30;
     a.nblocks = ((a.nbx - a.tailHalf) * a.nby + (a.tailHalf ? (a.nby + 1) / 2 : 0)) * a.nbz;
     if (outlet) {
         if (!outletFusable(L, *outlet)) throw std::invalid_argument("outlet face cannot be applied in the stage kernel");

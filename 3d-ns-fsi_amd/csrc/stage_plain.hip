@@ -40,11 +40,10 @@ struct SegSlot {
 __device__ __forceinline__ SegSlot segSlot(const Layout& L, int j, int k) {
     SegSlot r;
     const int t = threadIdx.x;
-    r.i = blockIdx.x * L.sn + t - L.sp;
+    r.i = blockIdx.x * Layout::kSegNodes + t - 1;
     r.o = ((size_t)k * L.NJ + j) * L.pitch + (size_t)blockIdx.x * Layout::kSegSlots + t;
-    r.pad = t < L.sp || t >= L.sp + L.sn || r.i >= L.NI;
-    // the trailing padding slots exist when the next segment does
-    r.write = t < L.sp || (int)blockIdx.x * L.sn + min(t - L.sp, L.sn) < L.NI;
+    r.pad = t == 0 || t == Layout::kSegSlots - 1 || r.i >= L.NI;
+    r.write = t == 0 || r.i < L.NI;  // slot 63 exists when the next segment does
     return r;
 }
 
@@ -136,7 +135,7 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
     const T* mu = prim + 5 * F;
     const T* ka = prim + 6 * F;
     // x-steps from column i (a step can cross the padding slots between segments)
-    const long XP = L.xstep(i, 1), XM = L.xstep(i, -1), Y = L.pitch, Z = (long)L.plane;
+    const long XP = Layout::xstep(i, 1), XM = Layout::xstep(i, -1), Y = L.pitch, Z = (long)L.plane;
     const size_t e_ = o + XP, w_ = o + XM, n_ = o + Y, s_ = o - Y, u_ = o + Z, d_ = o - Z;
     T kq[5];
     // continuity (src/Solver.cpp:191-194)
@@ -254,7 +253,7 @@ __global__ void k_rhs_plain(Layout L, Params P, const T* Sin, const T* prim, con
 template <class T>
 void launchDerive(hipStream_t s, const Layout& L, const Params& P, const T* S, T* prim, const uint8_t* cls, const Window& W,
                   const T* rhoPre, const T* rhoEPre, int mode) {
-    dim3 blk(Layout::kSegSlots), grd((L.NI + L.sn - 1) / L.sn, L.NJ, L.NK);
+    dim3 blk(Layout::kSegSlots), grd((L.NI + Layout::kSegNodes - 1) / Layout::kSegNodes, L.NJ, L.NK);
     if (mode == 1) hipLaunchKernelGGL((k_derive<T, 1>), grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
     else if (mode == 2) hipLaunchKernelGGL((k_derive<T, 2>), grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
     else hipLaunchKernelGGL((k_derive<T, 0>), grd, blk, 0, s, L, P, S, prim, cls, W, rhoPre, rhoEPre);
@@ -264,7 +263,7 @@ template <class T, bool K, bool LIT>
 void launchRhsPlainK(hipStream_t s, int stage, const Layout& L, const Params& P, const T* Sin, const T* prim, const T* U,
                      T* Sout, T* A, const uint8_t* cls, const double* dtdev, const T* fRho, const int32_t* fFlag,
                      int8_t* comp) {
-    dim3 blk(Layout::kSegSlots), grd((L.NI + L.sn - 1) / L.sn, L.NJ, L.NK);
+    dim3 blk(Layout::kSegSlots), grd((L.NI + Layout::kSegNodes - 1) / Layout::kSegNodes, L.NJ, L.NK);
     switch (stage) {
         case 1:
             hipLaunchKernelGGL((k_rhs_plain<T, 1, K, LIT>), grd, blk, 0, s, L, P, Sin, prim, U, Sout, A, cls, dtdev, fRho,
@@ -324,7 +323,7 @@ template <class T>
 __device__ __forceinline__ void rhoFilterNode(const Layout& L, const T* U, T* out, const uint8_t* cls, int i, int j, int k) {
     size_t o = L.off(i, j, k);
     if ((cls[o] & CLS_TYPE_MASK) != NS_FLUID_INTERIOR) return;
-    const long XP = L.xstep(i, 1), XM = L.xstep(i, -1), Y = L.pitch, Z = (long)L.plane;
+    const long XP = Layout::xstep(i, 1), XM = Layout::xstep(i, -1), Y = L.pitch, Z = (long)L.plane;
     // (6 rho_P + sum of the 6 face neighbours) / 12, in the reference's summation order
     out[o] = ((T)6 * U[o] + U[o + XP] + U[o + XM] + U[o + Y] + U[o - Y] + U[o + Z] + U[o - Z]) / (T)12;
 }

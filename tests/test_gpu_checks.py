@@ -319,10 +319,8 @@ def test_field_beyond_4gib_matches_a_thin_slab(precision, nk):
 
     big, thin = make(nk), make(5)
     eb = 4 if precision == "fp32" else 8
-    # device row: x-segments of sn nodes in 64 slots (FP32: 60, FP64: 62), padded to whole segments (FP32) or
-    # 128 bytes (FP64)
-    sn = 60 if precision == "fp32" else 62
-    slots = (big.NI - 1) + (64 - sn) * ((big.NI - 1) // sn) + (64 - sn) // 2 + 1
+    # device row: x-segments of 62 nodes in 64 slots, padded to whole segments (FP32) or 128 bytes (FP64)
+    slots = (big.NI - 1) + 2 * ((big.NI - 1) // 62) + 2
     unit = 64 if precision == "fp32" else 128 // eb
     field_bytes = nk * big.NJ * (-(-slots // unit) * unit) * eb
     print(f"{precision} 126x126x{nk}: {field_bytes / 2**30:.2f} GiB per field", flush=True)

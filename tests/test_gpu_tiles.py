@@ -28,7 +28,7 @@ def run(cfg, steps, **opts):
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_tile_shape_invariance(prec, N):
     # N = 41: one x-segment; the last z-chunk of every tile height has an odd plane count (9 or 41).
-    # N = 64: the last x-segment holds 2 (FP64) or 4 (FP32) nodes, so its tiles are half-width (two y-bands per wave), with an
+    # N = 64: the last x-segment holds 2 nodes, so its tiles are half-width (two y-bands per wave), with an
     # odd y-band count at 6 and 14 rows (the last band pair has one band).
     cfg = cases.sphere48(N=N, steps=4, precision=prec)
     ref = run(cfg, 4, tile_ty=8, tile_kc=64)
