@@ -1,6 +1,7 @@
-"""Precision study: FP32 forms of the time step against the FP64 form on one case.
+"""Precision study: FP32 forms of the time step against the FP64 form on one case, or on the batched sweep.
 
     python 3d-ns-fsi_amd/precision_study.py [--workload cylinder256] [--N 256] [--steps 1000] [--out FILE]
+    python 3d-ns-fsi_amd/precision_study.py --workload sweep64 [--steps 500] [--out FILE]
 
 Runs the case in FP64 (the form that matches the serial FP64 oracle to <= 1e-12 relative L2 per field,
 see tests/test_gpu_parity.py), in FP32 with the compensated combine and in FP32 with the plain combine,
@@ -8,6 +9,11 @@ each with the body-force record on, and reports for each FP32 form the relative 
 conserved field against FP64 after the last step, and the drag coefficient (last step and mean over the
 last tenth of the run) of every form. The FP64 device run is the reference at these sizes because the
 serial oracle would need hours for 256^3 x 1000 steps.
+
+With --workload sweep64 the 64 cases of the batched sweep run together in FP32 with the compensated combine
+(the batched form), each case also alone in FP64, and every case's relative L2 and Linf errors per conserved
+field are reported after the last step; cases whose FP64 run also goes non-finite within the steps (the
+reference's scheme loses them too) are listed as such.
 """
 import argparse
 import json
@@ -53,14 +59,88 @@ def errors(U, V):
     return out
 
 
+def sweep_study(a):
+    cfg32 = cases.sweep64(steps=a.steps, precision="fp32")
+    cfg64 = cases.sweep64(steps=a.steps, precision="fp64")
+    print(f"sweep64: batched FP32 (compensated combine), {a.steps} steps", flush=True)
+    t0 = time.perf_counter()
+    b = nsfsi.Batch(cfg32)
+    b.initialize()
+    b.setOption("graph", 1)
+    done = 0
+    while done < a.steps:
+        n = min(100, a.steps - done)
+        try:
+            b.step(n)
+        except nsfsi.NSError as e:  # a case went non-finite: the batch still advanced every case
+            if e.status != 3:
+                raise
+        done += n
+        print(f"  batch: {done}/{a.steps} steps", flush=True)
+    U32 = [b.conserved(c) for c in range(len(cfg32))]
+    wall32 = time.perf_counter() - t0
+    b.close()
+    rows, worst = [], (0.0, 0.0)
+    t0 = time.perf_counter()
+    for c, cfg in enumerate(cfg64):
+        s = nsfsi.Solver(cfg)
+        s.initialize()
+        s.setOption("graph", 1)
+        try:
+            s.marchTimeStep(a.steps)
+        except nsfsi.NSError as e:
+            if e.status != 3:
+                raise
+        U64 = s.mesh.conserved()
+        s.close()
+        fin32, fin64 = bool(np.all(np.isfinite(U32[c]))), bool(np.all(np.isfinite(U64)))
+        r = {"case": c, "N": cfg.NI, "radius": cfg.body_radius, "M": cfg.M_0, "finite_fp32": fin32,
+             "finite_fp64": fin64}
+        if fin32 and fin64:
+            errs = errors(U32[c], U64)
+            r["rel_l2"] = [e[0] for e in errs]
+            r["rel_linf"] = [e[1] for e in errs]
+            worst = (max(worst[0], max(r["rel_l2"])), max(worst[1], max(r["rel_linf"])))
+        rows.append(r)
+        if c % 8 == 7:
+            print(f"  FP64 cases: {c + 1}/64", flush=True)
+    wall64 = time.perf_counter() - t0
+    lines = [f"# Precision study: sweep64, 64 cases (grid edges 32-128), {a.steps} RK4 steps", "",
+             "Batched FP32 with the compensated combine (every case equal bit for bit to its single-case FP32 run)",
+             "against each case alone in FP64 (the form that matches the serial oracle within its one-ulp",
+             "sensitivity). Relative L2 = ||U - U64|| / ||U64||, relative Linf = max|U - U64| / max|U64|, per conserved",
+             "field after the last step; each case at its own step sizes in both forms.", "",
+             f"Wall time: batched FP32 {wall32:.1f} s, the 64 FP64 runs {wall64:.1f} s. Worst over the finite cases and",
+             f"fields: relative L2 {worst[0]:.3e}, relative Linf {worst[1]:.3e}.", "",
+             "| case | N | sphere r | Mach | " + " | ".join(f"{f} L2 / Linf" for f in nsfsi.FIELDS[:5]) + " |",
+             "|---|---|---|---|" + "---|" * 5]
+    for r in rows:
+        if "rel_l2" in r:
+            cells = " | ".join(f"{l2:.2e} / {li:.2e}" for l2, li in zip(r["rel_l2"], r["rel_linf"]))
+        else:
+            cells = " | ".join(["non-finite in " + ("FP32 and FP64" if not r["finite_fp64"] else "FP32 only")] * 5)
+        lines.append(f"| {r['case']} | {r['N']} | {r['radius']:.3f} | {r['M']:.3f} | {cells} |")
+    text = "\n".join(lines) + "\n"
+    print(text)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(text)
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump({"workload": "sweep64", "steps": a.steps, "cases": rows}, f, indent=1)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="cylinder256", choices=sorted(cases.WORKLOADS))
+    ap.add_argument("--workload", default="cylinder256", choices=sorted(cases.WORKLOADS) + ["sweep64"])
     ap.add_argument("--N", type=int, default=None)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--out", default=None, help="also write the report (markdown) here")
     ap.add_argument("--json", default=None, help="also write the figures as JSON here")
     a = ap.parse_args()
+    if a.workload == "sweep64":
+        sweep_study(a)
+        return
     kw = {"N": a.N} if a.N else {}
     forms = [("fp64", 0), ("fp32", 1), ("fp32", 0)]
     res = {}
