@@ -1,13 +1,15 @@
 """Adjoint study: gradients of the time-averaged drag coefficient against central finite differences.
 
     python 3d-ns-fsi_amd/adjoint_study.py [--N 96] [--steps 400] [--n0 300] [--n1 400] [--checkpoint 20]
-                                          [--directions 16] [--eps 1e-4] [--out FILE]
+                                          [--directions 16] [--eps 1e-4] [--precision fp64|fp32] [--out FILE]
 
-FP64 sphere case on [-2,2]^3 (r = 0.5 at the centre, Re_D = 100, M = 0.3, Gaussian perturbation 1e-3,
+Sphere case on [-2,2]^3 (r = 0.5 at the centre, Re_D = 100, M = 0.3, Gaussian perturbation 1e-3,
 seed 5). One forward + reverse run (ns_adjoint_drag) gives J and dJ/d(initial state), dJ/d(inlet profile)
 and dJ/dRe; each finite difference runs the forward solver twice at the same frozen step sequence
 (J = mean Fx / (0.5 M^2 pi r^2) over the window, from the recorded body force). Directions: random
 normal over all conserved fields of all nodes (max-norm 1), a coherent inlet-profile direction, and Re.
+With --precision fp32 the adjoint run's forward is FP32 (compensated combine) and its reverse sweep FP64;
+the finite differences stay FP64, at the FP32 run's step sizes.
 """
 import argparse
 import math
@@ -52,6 +54,8 @@ def main():
     ap.add_argument("--checkpoint", type=int, default=20)
     ap.add_argument("--directions", type=int, default=16)
     ap.add_argument("--eps", type=float, default=1e-4)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
+                    help="precision of the adjoint run's forward steps (the reverse sweep is FP64)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     assert a.n0 >= 1, "the forward-only evaluation records forces after steps (n0 >= 1)"
@@ -61,14 +65,18 @@ def main():
     U = s.mesh.conserved()
     q = np.ones(s.inletPlaneShape())
     s.close()
+    cfa = cfg
+    if a.precision == "fp32":
+        cfa = cases.sphere96(N=a.N, steps=a.steps, precision="fp32")
+        cfa.kahan = 1
     t0 = time.perf_counter()
-    r = nt.drag_and_gradients(cfg, U, q, cfg.Re, a.steps, a.n0, a.n1, a.checkpoint)
+    r = nt.drag_and_gradients(cfa, U, q, cfg.Re, a.steps, a.n0, a.n1, a.checkpoint)
     t_adj = time.perf_counter() - t0
     print(f"J = {r['J']:.12e}  (forward + reverse {t_adj:.1f} s)", flush=True)
     t0 = time.perf_counter()
     Jf = forward_J(cfg, U, q, cfg.Re, a.steps, a.n0, a.n1, r["dts"])
     t_fwd = time.perf_counter() - t0
-    print(f"forward-only J = {Jf:.12e} ({t_fwd:.1f} s)", flush=True)
+    print(f"forward-only FP64 J = {Jf:.12e} ({t_fwd:.1f} s)", flush=True)
     rows = []
     rng = np.random.default_rng(2026)
     for d in range(a.directions):
@@ -88,10 +96,13 @@ def main():
     fd = (forward_J(cfg, U, q, cfg.Re + h, a.steps, a.n0, a.n1, r["dts"]) -
           forward_J(cfg, U, q, cfg.Re - h, a.steps, a.n0, a.n1, r["dts"])) / (2 * h)
     rows.append(("Re", fd, r["grad_re"]))
-    lines = [f"# Adjoint study: sphere {a.N}^3, {a.steps} FP64 RK4 steps, J = mean C_D over levels {a.n0}-{a.n1}", "",
+    form = "FP64" if a.precision == "fp64" else "FP32 forward (compensated combine), FP64 reverse sweep,"
+    lines = [f"# Adjoint study: sphere {a.N}^3, {a.steps} RK4 steps, {form} J = mean C_D over levels {a.n0}-{a.n1}", "",
              f"Command (repository root, one MI355X): `python 3d-ns-fsi_amd/adjoint_study.py --N {a.N} --steps {a.steps} "
-             f"--n0 {a.n0} --n1 {a.n1} --checkpoint {a.checkpoint} --directions {a.directions} --eps {a.eps}`", "",
-             f"J = {r['J']:.12e} (adjoint run) / {Jf:.12e} (forward-only run at the same step sizes).",
+             f"--n0 {a.n0} --n1 {a.n1} --checkpoint {a.checkpoint} --directions {a.directions} --eps {a.eps}"
+             f"{' --precision fp32' if a.precision == 'fp32' else ''}`", "",
+             f"J = {r['J']:.12e} (adjoint run) / {Jf:.12e} (FP64 forward-only run at the same step sizes; "
+             f"relative difference {abs(r['J'] - Jf) / abs(Jf):.2e}).",
              f"Wall time: forward + checkpointed reverse {t_adj:.1f} s (checkpoint every {a.checkpoint} steps); "
              f"one forward run {t_fwd:.1f} s.", "",
              f"Central differences, eps = {a.eps} (FP64):", "",

@@ -1,5 +1,6 @@
 /*
- * adjoint.h — launchers of the discrete-adjoint kernels (adjoint.hip), FP64.
+ * adjoint.h — launchers of the discrete-adjoint kernels (adjoint.hip), FP64 (the forward run they reverse
+ * may be FP32: its states are widened first).
  */
 #pragma once
 
@@ -51,5 +52,7 @@ void launchAdjRoute(hipStream_t s, const Layout& L, const uint8_t* cls, const do
 void launchAxpyWindow(hipStream_t s, double* y, const double* x, size_t n);
 void launchAdjTablesToState(hipStream_t s, const Layout& L, const Window& W, const int64_t* fn, int64_t n,
                             const double* Pb, const double* PEb, double* Sb);
+// y[q] = x[q] for q < n: the stage states and tables of an FP32 forward step, widened for the FP64 reverse sweep.
+void launchWiden(hipStream_t s, const float* x, double* y, size_t n);
 
 }  // namespace nsfsi

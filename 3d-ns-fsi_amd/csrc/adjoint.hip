@@ -522,6 +522,11 @@ __global__ void k_adj_tables_to_state(Layout L, Window W, const int64_t* fn, int
     Sb[o + 4 * L.fstride] += PEb[w];
 }
 
+__global__ void k_widen(const float* x, double* y, size_t n) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) y[q] = (double)x[q];
+}
+
 inline unsigned nb(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -587,6 +592,10 @@ void launchAdjTablesToState(hipStream_t s, const Layout& L, const Window& W, con
                             const double* Pb, const double* PEb, double* Sb) {
     if (n > 0)
         hipLaunchKernelGGL(k_adj_tables_to_state, dim3(nb(n, 128)), dim3(128), 0, s, L, W, fn, n, Pb, PEb, Sb);
+}
+
+void launchWiden(hipStream_t s, const float* x, double* y, size_t n) {
+    if (n > 0) hipLaunchKernelGGL(k_widen, dim3(nb((int64_t)n, 256)), dim3(256), 0, s, x, y, n);
 }
 
 }  // namespace nsfsi

@@ -1,5 +1,5 @@
 /*
- * adjoint_host.cpp — discrete adjoint of the drag objective over a run of FP64 time steps.
+ * adjoint_host.cpp — discrete adjoint of the drag objective over a run of time steps.
  *
  * J = 1 / (n1 - n0 + 1) * sum_{L = n0..n1} C_D(U^L),  C_D = Fx / (0.5 M_0^2 A_ref), Fx the force on the
  * body of ns_force_history. The step sizes are those of the forward run, held fixed (the adjoint is that
@@ -7,7 +7,10 @@
  * returns). Gradients: with respect to the initial conserved state, the inlet profile factors and Re.
  *
  * Memory: checkpoints of the full state every `ckpt` steps; per segment a recomputation that records
- * the four stage states of every step (tape), then the reverse steps of the segment. Reverse step m
+ * the four stage states of every step (tape), then the reverse steps of the segment. An FP32 case runs the
+ * forward and the recomputation in FP32 (its step, compensated combine included; forces and step sizes
+ * reduced in FP64) and the reverse sweep in FP64 on the widened stage states: the adjoint of the FP64 map
+ * linearised along the FP32 trajectory. Reverse step m
  * (forward U^{m-1} -> U^m) consumes the adjoint of U^m and of its pre-filter tables and produces that of
  * U^{m-1}; stage 1 of step m copies its non-interior nodes from U^{m-2} (the reference's stale values),
  * so it also produces an adjoint of U^{m-2} that reverse step m-1 adds to its result.
@@ -37,7 +40,6 @@ double Case::referenceArea() const {
 void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* gradU0, double* gradInflow,
                        double* gradRe, double* dtsOut) {
     if (!initialized) throw StateError("adjoint before init");
-    if (fp32) throw Unsupported("the adjoint runs on the FP64 form (precision = fp64)");
     if (level != 0) throw StateError("the adjoint starts from the initial state (time level 0)");
     if (!filterDisabled()) throw Unsupported("the adjoint does not cover the periodic density filter");
     if (nGhost == 0) throw Unsupported("the drag objective needs an immersed body");
@@ -51,6 +53,9 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
     useGraph = false;
     const size_t F5 = 5 * L.fstride, WN = std::max<size_t>(1, W.size());
     const size_t fb = F5 * sizeof(double), wb = WN * sizeof(double);
+    // the state buffers and tables in the forward's precision, and the compensated combine's carry
+    const size_t es = fp32 ? sizeof(float) : sizeof(double), fbT = F5 * es, wbT = WN * es;
+    const size_t cb = comp ? F5 * sizeof(int8_t) : 0;
     std::vector<void*> mine;
     auto dalloc = [&](size_t bytes) {
         void* p = nullptr;
@@ -65,23 +70,46 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
     try {
         // ---- forward pass with checkpoints; records the step sizes and the forces
         const int nck = nsteps / ckpt + 1;
-        struct Ck { double *U, *A, *P, *PE; StepScalars s; };
+        // U64 / P64 / PE64: the state and tables in FP64 for the reverse sweep (for an FP64 case, U / P / PE)
+        struct Ck { void *U, *A, *P, *PE; int8_t* C; double *U64, *P64, *PE64; StepScalars s; };
         std::vector<Ck> cks(nck + 1);
         std::vector<double> tAt(nsteps + 1), dtAt(nsteps + 1);
         auto save = [&](Ck& c) {
-            if (!c.U) { c.U = dalloc(fb); c.A = dalloc(fb); c.P = dalloc(wb); c.PE = dalloc(wb); }
-            copy(c.U, U, fb);
-            copy(c.A, A, fb);
-            copy(c.P, rhoPre, wb);
-            copy(c.PE, rhoEPre, wb);
+            if (!c.U) {
+                c.U = dalloc(fbT);
+                c.A = dalloc(fbT);
+                c.P = dalloc(wbT);
+                c.PE = dalloc(wbT);
+                if (comp) c.C = (int8_t*)dalloc(cb);
+                if (fp32) {
+                    c.U64 = dalloc(fb);
+                    c.P64 = dalloc(wb);
+                    c.PE64 = dalloc(wb);
+                } else {
+                    c.U64 = (double*)c.U;
+                    c.P64 = (double*)c.P;
+                    c.PE64 = (double*)c.PE;
+                }
+            }
+            copy(c.U, U, fbT);
+            copy(c.A, A, fbT);
+            copy(c.P, rhoPre, wbT);
+            copy(c.PE, rhoEPre, wbT);
+            if (comp) copy(c.C, comp, cb);
+            if (fp32) {
+                launchWiden(stream, (const float*)U, c.U64, F5);
+                launchWiden(stream, (const float*)rhoPre, c.P64, WN);
+                launchWiden(stream, (const float*)rhoEPre, c.PE64, WN);
+            }
             hipCheck(hipMemcpyAsync(&c.s, scal, sizeof(StepScalars), hipMemcpyDeviceToHost, stream), "copy");
             hipCheck(hipStreamSynchronize(stream), "sync");
         };
         auto restore = [&](const Ck& c) {
-            copy(U, c.U, fb);
-            copy(A, c.A, fb);
-            copy(rhoPre, c.P, wb);
-            copy(rhoEPre, c.PE, wb);
+            copy(U, c.U, fbT);
+            copy(A, c.A, fbT);
+            copy(rhoPre, c.P, wbT);
+            copy(rhoEPre, c.PE, wbT);
+            if (comp) copy(comp, c.C, cb);
             hipCheck(hipMemcpyAsync(scal, &c.s, sizeof(StepScalars), hipMemcpyHostToDevice, stream), "copy");
             hipCheck(hipStreamSynchronize(stream), "sync");
             readScalars();
@@ -91,8 +119,12 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
         forceHost.clear();
         std::vector<double> fx(nsteps + 1, 0.0);
         if (n0 == 0) {  // force of the initial state
-            launchBodyForce<double>(stream, L, P, W, (const double*)U, cls, (const double*)rhoPre,
-                                    (const double*)rhoEPre, ghostList, nGhost, partials, scal, forceRing, histCap);
+            if (fp32)
+                launchBodyForce<float>(stream, L, P, W, (const float*)U, cls, (const float*)rhoPre,
+                                       (const float*)rhoEPre, ghostList, nGhost, partials, scal, forceRing, histCap);
+            else
+                launchBodyForce<double>(stream, L, P, W, (const double*)U, cls, (const double*)rhoPre,
+                                        (const double*)rhoEPre, ghostList, nGhost, partials, scal, forceRing, histCap);
             readScalars();
             fx[0] = forceHost[forceHost.size() - 3];
         }
@@ -100,7 +132,8 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
             if (lv % ckpt == 0) save(cks[lv / ckpt]);
             tAt[lv] = t;
             dtAt[lv] = dt;
-            stepOnce<double>();  // swaps U and A itself
+            if (fp32) stepOnce<float>();  // swaps U and A itself
+            else stepOnce<double>();
             hipCheck(hipGetLastError(), "forward step");
             readScalars();
             fx[lv + 1] = forceHost[forceHost.size() - 3];
@@ -190,7 +223,7 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
         zero(carryIn, fb);
         if (objAt(nsteps) != 0.0) {
             zero(lam[0], fb);
-            rhsT(last.U, last.P, last.PE, lam[0], objAt(nsteps), Ub, Pb, PEb);
+            rhsT(last.U64, last.P64, last.PE64, lam[0], objAt(nsteps), Ub, Pb, PEb);
         }
 
         // tape of one segment
@@ -206,7 +239,7 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
             for (int m = a0 + 1; m <= b0; ++m) stepTaped(&tape[m - a0 - 1]);
             for (int m = b0; m >= a0 + 1; --m) {
                 const StageTape& tp = tape[m - a0 - 1];
-                const double* endS = m == b0 ? (seg == (nsteps - 1) / ckpt ? last.U : cks[seg + 1].U)
+                const double* endS = m == b0 ? (seg == (nsteps - 1) / ckpt ? last.U64 : cks[seg + 1].U64)
                                              : tape[m - a0].S[0];
                 const double tt = tAt[m - 1], h = dtAt[m - 1];
                 // final boundary application of step m (produced U^m and its tables)

@@ -147,7 +147,7 @@ public:
         double* P[4];
         double* PE[4];
     };
-    // One FP64 step (kernel by kernel) recording its stage states into tape.
+    // One step (kernel by kernel) recording its stage states into tape, in FP64 (an FP32 case's widened).
     void stepTaped(StageTape* tape);
     // Discrete adjoint of J = mean over time levels n0..n1 of C_D after nsteps steps from the current
     // (initial) state, at the step sizes of the forward run (frozen); see ns_adjoint_drag.

@@ -17,8 +17,10 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
+#include "adjoint.h"
 #include "case.h"
 #include "device_common.h"
 #include "kernels.h"
@@ -672,10 +674,16 @@ void Case::stepOnce() {
     }
     auto tapeCopy = [&](int slot, const T* S) {
         if (!tapeNow) return;
-        const size_t fb = 5 * L.fstride * sizeof(T), wb = std::max<size_t>(1, W.size()) * sizeof(T);
-        hipCheck(hipMemcpyAsync(tapeNow->S[slot], S, fb, hipMemcpyDeviceToDevice, stream), "tape");
-        hipCheck(hipMemcpyAsync(tapeNow->P[slot], rhoPre, wb, hipMemcpyDeviceToDevice, stream), "tape");
-        hipCheck(hipMemcpyAsync(tapeNow->PE[slot], rhoEPre, wb, hipMemcpyDeviceToDevice, stream), "tape");
+        const size_t fn = 5 * L.fstride, wn = std::max<size_t>(1, W.size());
+        if constexpr (std::is_same<T, float>::value) {  // FP32 forward: the reverse sweep reads FP64 copies
+            launchWiden(stream, S, tapeNow->S[slot], fn);
+            launchWiden(stream, (const float*)rhoPre, tapeNow->P[slot], wn);
+            launchWiden(stream, (const float*)rhoEPre, tapeNow->PE[slot], wn);
+        } else {
+            hipCheck(hipMemcpyAsync(tapeNow->S[slot], S, fn * sizeof(T), hipMemcpyDeviceToDevice, stream), "tape");
+            hipCheck(hipMemcpyAsync(tapeNow->P[slot], rhoPre, wn * sizeof(T), hipMemcpyDeviceToDevice, stream), "tape");
+            hipCheck(hipMemcpyAsync(tapeNow->PE[slot], rhoEPre, wn * sizeof(T), hipMemcpyDeviceToDevice, stream), "tape");
+        }
     };
     tapeCopy(0, u);
     if (!literal()) primsStored = false;  // the other forms do not keep the stored primitive fields
@@ -1047,11 +1055,16 @@ void Case::step(int nsteps) {
 
 void Case::stepTaped(StageTape* tape) {
     if (!initialized) throw StateError("step before init");
-    if (fp32) throw Unsupported("the adjoint runs on the FP64 form");
     checkSchedule(1);
     syncStoredPrims();
     tapeNow = tape;
-    stepOnce<double>();  // swaps U and A itself
+    try {
+        if (fp32) stepOnce<float>();  // swaps U and A itself
+        else stepOnce<double>();
+    } catch (...) {
+        tapeNow = nullptr;
+        throw;
+    }
     tapeNow = nullptr;
     hipCheck(hipGetLastError(), "taped step");
     readScalars();

@@ -198,11 +198,13 @@ ns_status ns_set_dt_schedule(ns_handle* h, const double* dts, int64_t n);
 ns_status ns_set_inflow_profile(ns_handle* h, const double* q, int64_t n);
 
 /* Discrete adjoint of the drag objective (no reference counterpart). From the initial state (time level 0,
- * precision fp64, no periodic density filter, an immersed body), runs nsteps FP64 steps and evaluates
+ * no periodic density filter, an immersed body), runs nsteps steps in the case's precision and evaluates
  *   J = 1/(n1 - n0 + 1) * sum_{L=n0..n1} C_D(U^L),  C_D = Fx / (0.5 M_0^2 A_ref)
  * (Fx and A_ref as in ns_force_history), then the hand-written reverse sweep (transposed right-hand side,
  * combine and boundary application of every stage; checkpoints every `checkpoint` steps with
- * recomputation) at the step sizes of that run held fixed. Outputs (any may be null):
+ * recomputation) at the step sizes of that run held fixed. The reverse sweep is FP64 in both precisions:
+ * for an fp32 case it is the adjoint of the FP64 step linearised at the FP32 run's stage states (forces,
+ * J and step sizes reduced in FP64 either way). Outputs (any may be null):
  *   J; grad_u0 = dJ/d(initial conserved fields), 5 x-fastest arrays as ns_get_field;
  *   grad_inflow = dJ/d(inlet profile factors q), the inlet plane's size (see ns_set_inflow_profile);
  *   grad_re = dJ/dRe; dts = the nsteps + 1 step sizes by time level (pass them to ns_set_dt_schedule to

@@ -168,6 +168,46 @@ def test_fourth_order_differences():
         assert rel <= 5e-9
 
 
+def test_fp32_forward_adjoint():
+    """FP32 forward (compensated combine) with the FP64 reverse sweep, at the FP64 run's step sizes: the
+    gradients are those of the FP64 map linearised along the FP32 trajectory, so they agree with central
+    differences of the FP64 run and with the FP64 adjoint to the FP32 rounding of the states they are
+    evaluated at (bars 1e-4; measured <= 2.5e-8 against the differences, 1.7e-8 against the FP64 adjoint). Recomputation from checkpoints restores the FP32 state,
+    tables and carry exactly: every checkpoint interval gives the same J bit for bit and the same gradients
+    to 1e-12 (the transposed ghost interpolation adds into its donor nodes atomically, so the gradients are
+    reproducible to the order of those sums; a recomputation off by one FP32 rounding would move them by
+    orders of magnitude more)."""
+    cfg = small_case()
+    c32 = cases.sphere48(N=24, steps=10, precision="fp32")
+    c32.kahan = 1
+    U, q = base(cfg)
+    nsteps, n0, n1 = 12, 6, 12
+    r64 = nt.drag_and_gradients(cfg, U, q, cfg.Re, nsteps, n0, n1, 5)
+    r32 = nt.drag_and_gradients(c32, U, q, cfg.Re, nsteps, n0, n1, 5, dts=r64["dts"])
+    assert np.array_equal(r32["dts"], r64["dts"])
+    for ck in (1, nsteps):
+        o = nt.drag_and_gradients(c32, U, q, cfg.Re, nsteps, n0, n1, ck, dts=r64["dts"])
+        dg = [np.linalg.norm(o[k] - r32[k]) / np.linalg.norm(r32[k]) for k in ("grad_u0", "grad_inflow", "grad_re")]
+        print(f"  checkpoint {ck}: J {o['J']:.15e} gradients rel {dg}", flush=True)
+        assert o["J"] == r32["J"], ck
+        assert max(dg) <= 1e-12, (ck, dg)
+    relJ = abs(r32["J"] - r64["J"]) / abs(r64["J"])
+    g = np.linalg.norm(r32["grad_u0"] - r64["grad_u0"]) / np.linalg.norm(r64["grad_u0"])
+    gre = abs(r32["grad_re"] - r64["grad_re"]) / abs(r64["grad_re"])
+    print(f"fp32 forward: J rel {relJ:.2e}, grad_u0 rel L2 {g:.2e}, grad_re rel {gre:.2e}", flush=True)
+    assert relJ <= 1e-5 and g <= 1e-4 and gre <= 1e-4
+    rng = np.random.default_rng(32)
+    for _ in range(3):
+        d = rng.standard_normal(U.shape)
+        d /= np.abs(d).max()
+        fd = (J_at(cfg, U + EPS * d, q, cfg.Re, nsteps, n0, n1, r64["dts"]) -
+              J_at(cfg, U - EPS * d, q, cfg.Re, nsteps, n0, n1, r64["dts"])) / (2 * EPS)
+        ad = float(np.sum(r32["grad_u0"] * d))
+        rel = abs(fd - ad) / abs(fd)
+        print(f"  FP64 FD={fd:.10e} FP32-forward adjoint={ad:.10e} rel={rel:.2e}", flush=True)
+        assert rel <= 1e-4
+
+
 def periodic_sphere_adjoint():
     """A sphere in a box periodic in every direction (24x22x20, 6 steps, J over levels 3..6): the case, its
     initial state and the device adjoint's results."""

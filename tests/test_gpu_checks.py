@@ -260,11 +260,13 @@ def test_adjoint_sphere96_against_central_differences():
     assert worst <= 1e-4, worst
 
 
-def test_adjoint_objective_fp32_forward_against_fp64():
-    """The C5 objective evaluated with the FP32 forward (compensated combine; forces, reductions and ghost
-    solves in FP64) at the FP64 run's step sizes: J agrees with the FP64 J to 1e-5 relative. The adjoint
-    itself differentiates the FP64 step: the rounding of an FP32 forward (~1e-7 relative per step) is far
-    above the eps = 1e-4 central differences it would have to be checked against."""
+def test_adjoint_fp32_forward_against_fp64():
+    """The 96^3 adjoint with the FP32 forward (compensated combine; forces, J and step sizes reduced in FP64)
+    and the FP64 reverse sweep, at the FP64 run's step sizes: J agrees with the FP64 J to 1e-5 relative (also
+    as the FP32 force history of a plain run gives it), and the directional derivatives on the 16 random
+    initial-state directions and dJ/dRe agree with the FP64 adjoint's (pinned to central differences in the
+    test above) to 1e-4 relative (measured <= 7.4e-7; J: 3.8e-8). Central differences of the FP32 run itself are no check: its rounding
+    (~1e-7 relative per step) is far above what eps = 1e-4 differences resolve."""
     cfg = cases.sphere96(N=96, steps=400)
     s = solver(cfg)
     U = s.mesh.conserved()
@@ -273,7 +275,9 @@ def test_adjoint_objective_fp32_forward_against_fp64():
     r = nt.drag_and_gradients(cfg, U, q, cfg.Re, 400, 300, 400, 20)
     c32 = cases.sphere96(N=96, steps=400, precision="fp32")
     c32.kahan = 1
+    r32 = nt.drag_and_gradients(c32, U, q, cfg.Re, 400, 300, 400, 20, dts=r["dts"])
     s = solver(c32, body_force=1, graph=1)
+    s.setInitialField(U.reshape(-1))
     s.setDtSchedule(r["dts"])
     s.marchTimeStep(400)
     f = s.getForceHistory()
@@ -282,8 +286,22 @@ def test_adjoint_objective_fp32_forward_against_fp64():
     assert sel.sum() == 101
     J32 = float(f[sel, 1].mean() / (0.5 * c32.M_0 ** 2 * np.pi * c32.body_radius ** 2))
     rel = abs(J32 - r["J"]) / abs(r["J"])
-    print(f"J FP64 {r['J']:.12e} FP32 {J32:.12e} rel {rel:.2e}")
+    print(f"J FP64 {r['J']:.12e} FP32 {J32:.12e} (adjoint run {r32['J']:.12e}) rel {rel:.2e}")
     assert rel <= 1e-5, rel
+    assert abs(r32["J"] - J32) <= 1e-12 * abs(J32)
+    rng = np.random.default_rng(96)
+    worst = 0.0
+    for d in range(16):
+        dU = rng.standard_normal(U.shape)
+        dU /= np.abs(dU).max()
+        a64, a32 = float(np.sum(r["grad_u0"] * dU)), float(np.sum(r32["grad_u0"] * dU))
+        rel = abs(a32 - a64) / abs(a64)
+        print(f"direction {d}: FP64 adjoint {a64:.10e} FP32-forward adjoint {a32:.10e} rel {rel:.2e}", flush=True)
+        worst = max(worst, rel)
+    rel = abs(r32["grad_re"] - r["grad_re"]) / abs(r["grad_re"])
+    print(f"Re: FP64 adjoint {r['grad_re']:.10e} FP32-forward adjoint {r32['grad_re']:.10e} rel {rel:.2e}")
+    worst = max(worst, rel)
+    assert worst <= 1e-4, worst
 
 
 # ------------------------------------------------------------------ fields beyond 4 GiB
