@@ -475,6 +475,11 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
     const bool rhoFiltered = STAGE == 1 && a.fFlag && *a.fFlag;
     static_assert(!KAHAN || sizeof(T) == 4, "the compensated combine is the FP32 form");
     static_assert(!BUF || sizeof(T) == 4, "buffer-resource access is the FP32 form");
+    // FP32 stage 2 does not read U^n: stage 1 left S1 = U^n + (dt/2) k1 and the accumulator A = (dt/6) k1
+    // (compensated combine) or U^n + (dt/6) k1 (plain), so U^n = S1 - 3 A, or (3 A - S1) / 2, one rounding
+    // from the two values this stage loads anyway. Filter nodes load it: the boundary pass after stage 1
+    // replaced their S1 density and energy. FP64 keeps the load (its parity form).
+    constexpr bool rebuildU = sizeof(T) == 4 && STAGE == 2;
     const Mem<T, BUF> m(a, kb > 0 ? kb - 1 : 0);
     // clamped column of the stencil point (halo lanes and rows beyond the grid repeat the edge); the x-tile
     // is segment bx of the layout, so the halo lanes read the neighbouring segments
@@ -557,7 +562,7 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                 filterNode = (cl & CLS_FILTER) != 0;
             }
             if (STAGE == 2 || STAGE == 3 || (STAGE == 4 && KAHAN) || (SCAN && a.norms)) {
-                if (interior)
+                if (interior && (!rebuildU || filterNode))
                     for (int q = 0; q < 5; ++q) base[q] = m.ldU(q, bo);
             }
             if (STAGE >= 2) {
@@ -646,6 +651,10 @@ __device__ __forceinline__ void stageTile(const FusedArgs<T>& a, const int lb, S
                         }
                     } else if (STAGE == 2 || STAGE == 3) {
                         const T hs = (T)(STAGE == 2 ? dtd / 2 : dtd), h3 = (T)(dtd / 3);
+                        if (rebuildU && !filterNode)
+                            for (int q = 0; q < 5; ++q)
+                                base[q] = KAHAN ? fmaf(-3.0f, (float)acc[q], (float)cur[q])
+                                                : fmaf(1.5f, (float)acc[q], -0.5f * (float)cur[q]);
                         for (int q = 0; q < 5; ++q) {
                             vS[q] = base[q] + hs * rhs[q];
                             vA[q] = acc[q] + h3 * rhs[q];
