@@ -191,6 +191,10 @@ void Batch::init() {
     for (int p = 0; p < 2; ++p)
         for (int s = 0; s < 4; ++s) devArgs[p][s] = alloc(n * ab);
     devStageStart = (int*)alloc((n + 1) * sizeof(int));
+    if (!devCtr) {
+        devCtr = (unsigned*)alloc(9 * sizeof(unsigned));
+        hipCheck(hipMemsetAsync(devCtr, 0, 9 * sizeof(unsigned), stream), "memset");
+    }
     buildStageTables();
     initialized = true;
 }
@@ -251,7 +255,8 @@ void Batch::stepOnce() {
         launchRhoFilterBatched(stream, devViews, devFiltIds, devFiltStart, nFiltCases, filtNodeTotal, parity);
     }
     for (int st = 1; st <= 4; ++st) {
-        launchStageBatched(stream, st, tile.ty, true, kahan, devArgs[parity][st - 1], devStageStart, n, totalTiles);
+        launchStageBatched(stream, st, tile.ty, true, kahan, devArgs[parity][st - 1], devStageStart, n, totalTiles,
+                           persistent ? devCtr : nullptr);
         if (st == 1 && nFiltCases > 0)
             launchRhoCommitBatched(stream, devViews, devFiltIds, devFiltStart, nFiltCases, filtNodeTotal, parity);
         applyBCs(outRole[st - 1], frac[st - 1]);
@@ -320,7 +325,7 @@ double Batch::autotune(int reps) {
                 for (int r = 0; r < count; ++r)
                     for (int st = 1; st <= 3; ++st)
                         launchStageBatched(stream, st, tile.ty, false, kahan, devArgs[parity][st - 1], devStageStart, n,
-                                           totalTiles);
+                                           totalTiles, persistent ? devCtr : nullptr);
             };
             run(1);
             hipCheck(hipEventRecord(e0, stream), "record");
@@ -359,6 +364,8 @@ void Batch::setOption(const std::string& name, int64_t value) {
         if (value < kMinTileKc || value > 4096) throw std::invalid_argument("tile_kc out of range (8..4096)");
         tile.kc = (int)value;
         if (initialized) buildStageTables();
+    } else if (name == "persistent") {
+        persistent = value != 0;
     } else if (name == "autotune") {
         autotune(value > 0 ? (int)value : 3);
     } else {

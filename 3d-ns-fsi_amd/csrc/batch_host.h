@@ -30,6 +30,7 @@ public:
     std::vector<double> tuneTimes;
     bool useGraph = true;
     bool kahan = true;
+    bool persistent = false;  // stage launches in the persistent form (tiles pulled through devCtr)
     int totalTiles = 0;
 
 private:
@@ -47,6 +48,7 @@ private:
     CaseView<float>* devViews = nullptr;
     void* devArgs[2][4] = {};
     int* devStageStart = nullptr;
+    unsigned* devCtr = nullptr;  // tile counters of the persistent stage launches (zero between launches)
     int64_t* devFaceStart[6] = {};
     int64_t faceTotal[6] = {};
     std::vector<EdgeFaceListB> faceGroups;  // faces launched together, in application order

@@ -99,8 +99,10 @@ int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const
                    const double* dtdev, FusedTile tile, double* partials, int norms, int8_t* comp,
                    const EdgeFace* outlet = nullptr, const float* filteredRho = nullptr,
                    const int32_t* filterFlag = nullptr);
+// ctr != null: the persistent form (one block per resident slot pulling tiles through ctr, 9 unsigned
+// counters that must be zero before the first launch and that each launch leaves zero).
 void launchStageBatched(hipStream_t s, int stage, int ty, bool scan, bool kahan, const void* tab, const int* start,
-                        int ncases, int total);
+                        int ncases, int total, unsigned* ctr = nullptr);
 // Force on the body from the state S after a step's boundary application (FP64; the discrete momentum
 // balance over the faces between fluid interior and ghost nodes): appends (timeLevel, Fx, Fy, Fz) to
 // `ring` (cap entries of 4) and advances st->forceCount. partials: bodyForceBlocks(nGhost) x 3 doubles.

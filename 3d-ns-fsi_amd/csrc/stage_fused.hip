@@ -35,6 +35,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 
 #include "device_common.h"
@@ -803,6 +805,48 @@ void k_stage_batched(const FusedArgs<T>* __restrict__ tab, const int* __restrict
     stageTile<T, STAGE, TY, SCAN, KAHAN, sizeof(T) == 4>(tab[lo], g - start[lo], sm, wred);
 }
 
+// The same stage as a persistent launch: gridDim.x = the blocks the GPU holds at once (a multiple of 8).
+// The tiles are split into 8 contiguous ranges as above, one per XCD; a block takes the next tile of its
+// own XCD's range (hardware block p runs on XCD p % 8) and, once that is exhausted, the next tiles of the
+// other ranges, so no XCD idles while another still has tiles. ctr[x] counts the tiles taken from range x,
+// ctr[8] the blocks finished; the last block to finish zeroes all nine for the next launch (no memset
+// node in a captured step). Every block leaves the loop once all ranges are exhausted.
+template <class T, int STAGE, int TY, bool SCAN, bool KAHAN>
+__global__ __launch_bounds__(64 * (TY + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+void k_stage_batched_persistent(const FusedArgs<T>* __restrict__ tab, const int* __restrict__ start, int ncases,
+                                int total, unsigned* __restrict__ ctr) {
+    __shared__ Smem<T, TY> sm[smemBuffers<T>()];
+    __shared__ double wred[TY + 2][10];
+    __shared__ int taken;
+    const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
+    const int per = (total + 7) / 8;
+    for (int r = 0; r < 8; ++r) {
+        const int x = (int)((blockIdx.x + r) & 7);
+        const int lo0 = x * per, n = min(per, total - lo0);
+        while (n > 0) {
+            if (leader) taken = (int)atomicAdd(&ctr[x], 1u);
+            __syncthreads();
+            const int t = taken;
+            __syncthreads();  // every thread has read `taken` before the leader writes it again
+            if (t >= n) break;
+            const int g = lo0 + t;
+            int lo = 0, hi = ncases - 1;  // last case whose first tile is <= g
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (start[mid] <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            stageTile<T, STAGE, TY, SCAN, KAHAN, sizeof(T) == 4>(tab[lo], g - start[lo], sm, wred);
+            __syncthreads();  // this tile's LDS readers are done before the next tile's prologue writes it
+        }
+    }
+    if (leader) {
+        __threadfence();
+        if (atomicAdd(&ctr[8], 1u) == gridDim.x - 1)
+            for (int q = 0; q < 9; ++q) atomicExch(&ctr[q], 0u);
+    }
+}
+
 // ---- force on the body: the discrete momentum balance of the flux form. Summed over all fluid interior
 // nodes, the right-hand sides telescope to the face fluxes across the outer boundary and across the faces
 // between interior nodes and ghost nodes; the latter are the momentum per unit time the body gives the
@@ -966,11 +1010,48 @@ void launchTY(hipStream_t s, int stage, const FusedArgs<T>& a, bool buf) {
     }
 }
 
+// Blocks of `kern` (threads per block) that the device holds at once, rounded down to a multiple of 8 (one
+// per XCD): the grid of a persistent launch. Found once per kernel and device.
+int residentBlocks(const void* kern, int threads) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> known;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = known.find({kern, dev});
+    if (it != known.end()) return it->second;
+    hipDeviceProp_t prop;
+    int perCU = 0;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, threads, 0) != hipSuccess)
+        throw std::runtime_error("occupancy query failed");
+    const int n = std::max(8, perCU * prop.multiProcessorCount / 8 * 8);
+    known[{kern, dev}] = n;
+    return n;
+}
+
 template <class T, int TY, bool K>
 void launchBatchedTYK(hipStream_t s, int stage, bool scan, const FusedArgs<T>* tab, const int* start, int ncases,
-                      int total) {
-    const int grid = (total + 7) / 8 * 8;
+                      int total, unsigned* ctr) {
     dim3 blk(64, TY + 2);
+    if (ctr) {
+        // persistent: as many blocks as the GPU holds at once (found once per kernel), a multiple of 8
+        auto launchP = [&](auto kern) {
+            const int resident = residentBlocks(reinterpret_cast<const void*>(kern), 64 * (TY + 2));
+            hipLaunchKernelGGL(kern, dim3(resident), blk, 0, s, tab, start, ncases, total, ctr);
+        };
+        switch (stage) {
+            case 1: launchP(k_stage_batched_persistent<T, 1, TY, false, K>); break;
+            case 2: launchP(k_stage_batched_persistent<T, 2, TY, false, K>); break;
+            case 3: launchP(k_stage_batched_persistent<T, 3, TY, false, K>); break;
+            default:
+                if (scan) launchP(k_stage_batched_persistent<T, 4, TY, true, K>);
+                else launchP(k_stage_batched_persistent<T, 4, TY, false, K>);
+                break;
+        }
+        return;
+    }
+    const int grid = (total + 7) / 8 * 8;
     switch (stage) {
         case 1: hipLaunchKernelGGL((k_stage_batched<T, 1, TY, false, K>), dim3(grid), blk, 0, s, tab, start, ncases, total); break;
         case 2: hipLaunchKernelGGL((k_stage_batched<T, 2, TY, false, K>), dim3(grid), blk, 0, s, tab, start, ncases, total); break;
@@ -1068,17 +1149,17 @@ int buildFusedArgs(void* dst, int stage, const Layout& L, const Params& P, const
 }
 
 void launchStageBatched(hipStream_t s, int stage, int ty, bool scan, bool kahan, const void* tab, const int* start,
-                        int ncases, int total) {
+                        int ncases, int total, unsigned* ctr) {
     const FusedArgs<float>* t = static_cast<const FusedArgs<float>*>(tab);
     switch (normTY(ty)) {
-        case 4: kahan ? launchBatchedTYK<float, 4, true>(s, stage, scan, t, start, ncases, total)
-                      : launchBatchedTYK<float, 4, false>(s, stage, scan, t, start, ncases, total); break;
-        case 6: kahan ? launchBatchedTYK<float, 6, true>(s, stage, scan, t, start, ncases, total)
-                      : launchBatchedTYK<float, 6, false>(s, stage, scan, t, start, ncases, total); break;
-        case 14: kahan ? launchBatchedTYK<float, 14, true>(s, stage, scan, t, start, ncases, total)
-                       : launchBatchedTYK<float, 14, false>(s, stage, scan, t, start, ncases, total); break;
-        default: kahan ? launchBatchedTYK<float, 8, true>(s, stage, scan, t, start, ncases, total)
-                       : launchBatchedTYK<float, 8, false>(s, stage, scan, t, start, ncases, total); break;
+        case 4: kahan ? launchBatchedTYK<float, 4, true>(s, stage, scan, t, start, ncases, total, ctr)
+                      : launchBatchedTYK<float, 4, false>(s, stage, scan, t, start, ncases, total, ctr); break;
+        case 6: kahan ? launchBatchedTYK<float, 6, true>(s, stage, scan, t, start, ncases, total, ctr)
+                      : launchBatchedTYK<float, 6, false>(s, stage, scan, t, start, ncases, total, ctr); break;
+        case 14: kahan ? launchBatchedTYK<float, 14, true>(s, stage, scan, t, start, ncases, total, ctr)
+                       : launchBatchedTYK<float, 14, false>(s, stage, scan, t, start, ncases, total, ctr); break;
+        default: kahan ? launchBatchedTYK<float, 8, true>(s, stage, scan, t, start, ncases, total, ctr)
+                       : launchBatchedTYK<float, 8, false>(s, stage, scan, t, start, ncases, total, ctr); break;
     }
 }
 

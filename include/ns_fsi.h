@@ -296,7 +296,8 @@ ns_status ns_batch_get_field(ns_batch* b, int32_t c, int32_t which, double* out)
 ns_status ns_batch_norm_history(ns_batch* b, int32_t c, int32_t var, double* out, int64_t n, int64_t* count);
 /* Options: "graph" (1 default), "tile_ty" / "tile_kc" (tile shape of the batched stage, shared by all
  * cases), "autotune" (time every tile shape on this batch with `value` launches each and keep the
- * fastest; states unchanged). */
+ * fastest; states unchanged), "persistent" (0 default: one block per tile; 1: one block per resident
+ * slot, each XCD's blocks pulling tiles from their own range and then from the others'; same results). */
 ns_status ns_batch_set_option(ns_batch* b, const char* name, int64_t value);
 ns_status ns_batch_get_tile(ns_batch* b, int32_t* ty, int32_t* kc);
 const char* ns_batch_last_error(ns_batch* b);

@@ -72,6 +72,27 @@ def test_batch_equals_single_cases(graph):
     b.close()
 
 
+@pytest.mark.parametrize("graph", [1, 0])
+def test_persistent_stage_launches_equal_grid_launches(graph):
+    """The persistent form (resident blocks pulling tiles through counters that each launch leaves at zero)
+    computes every tile as the grid launch does: equal states and times over several steps and graph
+    replays, so the counters come back to zero after every launch."""
+    cfgs = small_sweep(n=5, steps=7)
+    out = []
+    for persistent in (0, 1):
+        b = nsfsi.Batch(cfgs)
+        b.initialize()
+        b.setOption("graph", graph)
+        b.setOption("persistent", persistent)
+        b.step(2)
+        b.step(5)
+        out.append([(b.conserved(c), b.time(c)) for c in range(len(cfgs))])
+        b.close()
+    for c, ((U0, t0), (U1, t1)) in enumerate(zip(out[0], out[1])):
+        assert t0 == t1, c
+        assert np.array_equal(U0, U1), c
+
+
 def test_batch_cases_advance_at_their_own_dt():
     cfgs = small_sweep(n=4, steps=5)
     b = nsfsi.Batch(cfgs)
