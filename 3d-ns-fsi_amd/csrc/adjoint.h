@@ -6,6 +6,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+#include <utility>
+#include <vector>
+
 #include "kernels.h"
 #include "ns_internal.h"
 
@@ -39,13 +43,30 @@ struct AdjArgs {
 int adjRhsPartials(const Layout& L);  // dJ/dRe partial slots written by launchAdjRhs
 void launchAdjRhs(hipStream_t s, const AdjArgs& a, double* rePartial);
 
+// Fixed-order sums of the transposed boundary pieces (each sum in list order, so results do not depend
+// on thread scheduling): target t, at layout offset tgt[t], adds the entries src[start[t] .. start[t+1]-1].
+struct GatherPlan {
+    int64_t n = 0;
+    int64_t* tgt = nullptr;
+    int64_t* start = nullptr;
+    int32_t* src = nullptr;
+};
+// From (target offset, entry) pairs, in (target, entry) order; device arrays from alloc.
+GatherPlan buildGatherPlan(std::vector<std::pair<int64_t, int32_t>> pairs, const std::function<void*(size_t)>& alloc);
+
 // Transposed boundary pieces. Sb is updated in place (it holds the adjoint of the state after the
 // piece on entry and of the state before it on return).
-void launchAdjGhost(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, double* Sb);
+// Ghosts: contrib holds 40 doubles per ghost; donors lists, per fluid donor node, the entries 8 g + c of
+// the ghosts g whose interpolation box has it as corner c.
+void launchAdjGhost(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, double* Sb,
+                    double* contrib, const GatherPlan& donors);
+// Immersed filter: tmp holds 4 doubles per filter node; targets lists, per filter node or neighbour of
+// one, the entries 2 q + 1 (filter node q itself) and 2 q (a filter node q it neighbours).
 void launchAdjFilter(hipStream_t s, const Layout& L, const Window& W, const int64_t* fn, int64_t n, double* Sb,
-                     double* tmp, const double* Pb, const double* PEb);
+                     double* tmp, const double* Pb, const double* PEb, const GatherPlan& targets);
+// wbuf: one double per node of an inlet face (unused for the other face types).
 void launchAdjEdge(hipStream_t s, const Layout& L, const Params& P, const EdgeFace& f, double* Sb, const double* S,
-                   double ts, double* qb);
+                   double ts, double* qb, double* wbuf);
 // Interior nodes: Ub += B, lam[r] += coef[r] * B (null entries skipped); other nodes: prev += B (if set).
 void launchAdjRoute(hipStream_t s, const Layout& L, const uint8_t* cls, const double* B, double* Ub, double* const lam[4],
                     const double coef[4], double* prev);

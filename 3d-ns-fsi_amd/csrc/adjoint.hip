@@ -24,6 +24,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
+#include <stdexcept>
+#include <utility>
+#include <vector>
 
 #include "adjoint.h"
 #include "device_common.h"
@@ -301,12 +305,12 @@ __global__ void k_adj_rhs_b(AdjArgs a, double* rePartial) {
 
 // ---- boundary application, transposed
 
-__device__ __forceinline__ void atomAdd(double* p, double v) {
-    if (v != 0.0) atomicAdd(p, v);
-}
 
 // Ghost update: S[g] = +-(weights . corners); transposed: corners += weights * (+-Sbar[g]), Sbar[g] = 0.
-__global__ void k_adj_ghost(Layout L, Params P, GhostTables G, int64_t n, double* Sb) {
+// Phase 1 (here): every ghost writes its 8 corners' contributions (5 fields each) to contrib and clears
+// its own adjoint; phase 2 (k_adj_gather5 over the donor plan) adds them up per donor node in a fixed
+// order, so the result does not depend on the order the ghosts run in.
+__global__ void k_adj_ghost(Layout L, Params P, GhostTables G, int64_t n, double* Sb, double* contrib) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const size_t F = L.fstride;
@@ -350,33 +354,35 @@ __global__ void k_adj_ghost(Layout L, Params P, GhostTables G, int64_t n, double
         }
     }
     for (int c = 0; c < 8; ++c) {
-        if ((mask >> c) & 1) continue;  // ghost corners enter as zero
-        const size_t oc = (size_t)G.corner[8 * g + c];
-        atomAdd(&Sb[oc], w[1][c] * ib[0]);
-        atomAdd(&Sb[oc + F], w[0][c] * ib[1]);
-        atomAdd(&Sb[oc + 2 * F], w[0][c] * ib[2]);
-        atomAdd(&Sb[oc + 3 * F], w[0][c] * ib[3]);
-        atomAdd(&Sb[oc + 4 * F], w[1][c] * ib[4]);
+        double* out = contrib + (size_t)(8 * g + c) * 5;
+        const bool solid = (mask >> c) & 1;  // ghost corners enter as zero (the donor plan leaves them out)
+        out[0] = solid ? 0.0 : w[1][c] * ib[0];
+        out[1] = solid ? 0.0 : w[0][c] * ib[1];
+        out[2] = solid ? 0.0 : w[0][c] * ib[2];
+        out[3] = solid ? 0.0 : w[0][c] * ib[3];
+        out[4] = solid ? 0.0 : w[1][c] * ib[4];
     }
 }
 
-// Immersed filter, phase 1: take the adjoint of the filtered density and energy of every filter node
-// (tmp) and clear it in Sbar.
-__global__ void k_adj_filter_take(Layout L, const int64_t* fn, int64_t n, double* Sb, double* tmp) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const int64_t m = fn[q];
-    const int i = (int)(m % L.NI), j = (int)((m / L.NI) % L.NJ), k = (int)(m / ((int64_t)L.NI * L.NJ));
-    const size_t o = L.off(i, j, k);
-    tmp[2 * q] = Sb[o];
-    tmp[2 * q + 1] = Sb[o + 4 * L.fstride];
-    Sb[o] = 0.0;
-    Sb[o + 4 * L.fstride] = 0.0;
+// Phase 2 of a transposed gather: target t (layout offset p.tgt[t]) adds the entries p.start[t] ..
+// p.start[t+1]-1 of its list, five fields each, in list order.
+__global__ void k_adj_gather5(Layout L, GatherPlan p, const double* contrib, double* Sb) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= p.n) return;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int64_t e = p.start[t]; e < p.start[t + 1]; ++e) {
+        const double* c = contrib + (size_t)p.src[e] * 5;
+        for (int f = 0; f < 5; ++f) acc[f] += c[f];
+    }
+    const size_t o = (size_t)p.tgt[t];
+    for (int f = 0; f < 5; ++f) Sb[o + f * L.fstride] += acc[f];
 }
 
-// Phase 2: (6 P + sum of 6 neighbours) / 12 transposed, plus the pre-filter table adjoints (the tables
-// hold the pre-filter values of the filter nodes).
-__global__ void k_adj_filter_give(Layout L, Window W, const int64_t* fn, int64_t n, double* Sb, const double* tmp,
+// Immersed filter, (6 P + sum of 6 neighbours) / 12 transposed, plus the pre-filter table adjoints (the
+// tables hold the pre-filter values of the filter nodes). Phase 1: every filter node takes the adjoint of
+// its filtered density and energy, clears it, and writes what it gives its own node (6/12 of it plus the
+// tables' adjoints) and each of its six neighbours (1/12): tmp[4 q .. 4 q + 3].
+__global__ void k_adj_filter_take(Layout L, Window W, const int64_t* fn, int64_t n, double* Sb, double* tmp,
                                   const double* Pb, const double* PEb) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -384,21 +390,40 @@ __global__ void k_adj_filter_give(Layout L, Window W, const int64_t* fn, int64_t
     const int i = (int)(m % L.NI), j = (int)((m / L.NI) % L.NJ), k = (int)(m / ((int64_t)L.NI * L.NJ));
     const size_t o = L.off(i, j, k);
     const size_t E4 = 4 * L.fstride;
-    const long sy = L.pitch, sz = (long)L.plane;
-    const double br = tmp[2 * q] / 12.0, be = tmp[2 * q + 1] / 12.0;
+    const double br = Sb[o] / 12.0, be = Sb[o + E4] / 12.0;
     const size_t w = W.idx(i, j, k);
-    atomAdd(&Sb[o], 6.0 * br + Pb[w]);
-    atomAdd(&Sb[o + E4], 6.0 * be + PEb[w]);
-    const long nbs[6] = {Layout::xstep(i, 1), Layout::xstep(i, -1), sy, -sy, sz, -sz};
-    for (int r = 0; r < 6; ++r) {
-        atomAdd(&Sb[o + nbs[r]], br);
-        atomAdd(&Sb[o + nbs[r] + E4], be);
+    tmp[4 * q] = 6.0 * br + Pb[w];
+    tmp[4 * q + 1] = 6.0 * be + PEb[w];
+    tmp[4 * q + 2] = br;
+    tmp[4 * q + 3] = be;
+    Sb[o] = 0.0;
+    Sb[o + E4] = 0.0;
+}
+
+// Phase 2: target t adds, in list order, the entries of its list: 2 q + 1 for filter node q itself,
+// 2 q for a filter node q it neighbours.
+__global__ void k_adj_filter_gather(Layout L, GatherPlan p, const double* tmp, double* Sb) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= p.n) return;
+    double r = 0, e = 0;
+    for (int64_t x = p.start[t]; x < p.start[t + 1]; ++x) {
+        const int64_t q = p.src[x] >> 1;
+        const int own = p.src[x] & 1;
+        r += tmp[4 * q + (own ? 0 : 2)];
+        e += tmp[4 * q + (own ? 1 : 3)];
     }
+    const size_t o = (size_t)p.tgt[t];
+    Sb[o] += r;
+    Sb[o + 4 * L.fstride] += e;
 }
 
 // One edge face, transposed (src/Boundary.cpp:250-417 forward). S holds the forward values after the
-// boundary application (linearisation point of the inlet and outlet); qb the inlet profile adjoint.
-__global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const double* S, double ts, double* qb) {
+// boundary application (linearisation point of the inlet and outlet); qb the inlet profile adjoint. The
+// periodic, symmetry and outlet faces map distinct face nodes to distinct source nodes, so each target
+// has one contributor per launch; the inlet's in-plane filter spreads onto neighbouring columns and is
+// summed by k_adj_inlet_gather.
+__global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const double* S, double ts, double* qb,
+                           double* wbuf) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
     const int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
     const int lo[3] = {f.i0, f.j0, f.k0}, hi[3] = {f.i1, f.j1, f.k1};
@@ -424,11 +449,11 @@ __global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const dou
         const int Nn = f.axis == 0 ? L.NI : (f.axis == 1 ? L.NJ : L.NK);
         cc[f.axis] = f.side == 0 ? Nn - 2 : 1;
         const size_t src = L.off(cc[0], cc[1], cc[2]);
-        for (int r = 0; r < 5; ++r) atomAdd(&Sb[src + r * F], ob[r]);
+        for (int r = 0; r < 5; ++r) Sb[src + r * F] += ob[r];
     } else if (f.type == NS_BC_SYMMETRY) {
         // conserved mirror of the node two in: the normal momentum changes sign, all else is copied
         const size_t a2 = o + s2;
-        for (int r = 0; r < 5; ++r) atomAdd(&Sb[a2 + r * F], r == 1 + f.axis ? -ob[r] : ob[r]);
+        for (int r = 0; r < 5; ++r) Sb[a2 + r * F] += r == 1 + f.axis ? -ob[r] : ob[r];
     } else if (f.type == NS_BC_OUTLET) {
         // r, m extrapolated (2 x1 - x2); rhoE = |m|^2 / (2 (1 + r)) (p' = 0)
         const double r = S[o], mu_ = S[o + F], mv = S[o + 2 * F], mw = S[o + 3 * F];
@@ -437,11 +462,11 @@ __global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const dou
         const double gr = ob[0] - 0.5 * K * ob[4];
         const double gm[3] = {ob[1] + ob[4] * mu_ * ir, ob[2] + ob[4] * mv * ir, ob[3] + ob[4] * mw * ir};
         const size_t x1 = o + s1, x2 = o + s2;
-        atomAdd(&Sb[x1], 2.0 * gr);
-        atomAdd(&Sb[x2], -gr);
+        Sb[x1] += 2.0 * gr;
+        Sb[x2] += -gr;
         for (int r = 0; r < 3; ++r) {
-            atomAdd(&Sb[x1 + (1 + r) * F], 2.0 * gm[r]);
-            atomAdd(&Sb[x2 + (1 + r) * F], -gm[r]);
+            Sb[x1 + (1 + r) * F] += 2.0 * gm[r];
+            Sb[x2 + (1 + r) * F] += -gm[r];
         }
     } else {  // inlet
         const double acc = S[o];  // the boundary density equals the filtered extrapolation
@@ -459,24 +484,44 @@ __global__ void k_adj_edge(Layout L, Params P, EdgeFace f, double* Sb, const dou
         const double bvel = ob[1 + f.axis] * (1.0 + acc) + ob[4] * (1.0 + acc) * vel;
         if (qb && P.inflow) qb[qi] += bvel * ramp;  // one face node per profile entry: no race
         else if (qb) qb[(size_t)c[ax2] * (ax1 == 0 ? L.NI : L.NJ) + c[ax1]] += bvel * ramp;
-        // acc = (sum_nb extrap(nb) + n extrap(o)) / (2 n), extrap(x) = 2 rho(x + s1) - rho(x + 2 s1)
-        unsigned nbn = 0;
-        long nbo[4];
-        for (int ax : {ax1, ax2}) {
-            const long up = ax == 0 ? Layout::xstep(c[0], 1) : (ax == 1 ? (long)L.pitch : (long)L.plane);
-            const long down = ax == 0 ? Layout::xstep(c[0], -1) : -up;
-            if (c[ax] < hi[ax]) nbo[nbn++] = up;
-            if (c[ax] > lo[ax]) nbo[nbn++] = down;
-        }
-        const double wnb = bacc / (2.0 * nbn);
-        const double wo = bacc * (double)nbn / (2.0 * nbn);
-        atomAdd(&Sb[o + s1], 2.0 * wo);
-        atomAdd(&Sb[o + s2], -wo);
-        for (unsigned r = 0; r < nbn; ++r) {
-            atomAdd(&Sb[o + nbo[r] + s1], 2.0 * wnb);
-            atomAdd(&Sb[o + nbo[r] + s2], -wnb);
-        }
+        // acc = (sum_nb extrap(nb) + n extrap(o)) / (2 n), extrap(x) = 2 rho(x + s1) - rho(x + 2 s1): the
+        // columns of this node and its in-plane neighbours take their shares in k_adj_inlet_gather
+        const int n1 = hi[ax1] - lo[ax1] + 1;
+        wbuf[(size_t)(c[ax2] - lo[ax2]) * n1 + (c[ax1] - lo[ax1])] = bacc;
     }
+}
+
+// Inlet, phase 2: the column of face node o (the nodes one and two in) gathers, in a fixed order, the
+// extrapolation adjoints that the in-plane filter of o and of each of its in-plane neighbours p spread
+// onto it: bacc(o) n_o / (2 n_o) and bacc(p) / (2 n_p), n the node's in-plane neighbour count.
+__global__ void k_adj_inlet_gather(Layout L, EdgeFace f, double* Sb, const double* wbuf) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    const int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
+    const int lo[3] = {f.i0, f.j0, f.k0}, hi[3] = {f.i1, f.j1, f.k1};
+    const int n1 = hi[ax1] - lo[ax1] + 1, n2 = hi[ax2] - lo[ax2] + 1;
+    if (a >= n1 || b >= n2) return;
+    auto count = [&](int u, int v) {
+        return (u > 0) + (u < n1 - 1) + (v > 0) + (v < n2 - 1);
+    };
+    const int no = count(a, b);
+    double total = wbuf[(size_t)b * n1 + a] * (double)no / (2.0 * no);
+    const int du[4] = {1, -1, 0, 0}, dv[4] = {0, 0, 1, -1};
+    for (int r = 0; r < 4; ++r) {
+        const int u = a + du[r], v = b + dv[r];
+        if (u < 0 || u >= n1 || v < 0 || v >= n2) continue;
+        total += wbuf[(size_t)v * n1 + u] / (2.0 * count(u, v));
+    }
+    int c[3];
+    c[f.axis] = lo[f.axis];
+    c[ax1] = lo[ax1] + a;
+    c[ax2] = lo[ax2] + b;
+    const size_t o = L.off(c[0], c[1], c[2]);
+    const int dn = f.side == 0 ? 1 : -1;
+    const long st = f.axis == 1 ? (long)L.pitch : (long)L.plane;
+    const long s1 = f.axis == 0 ? Layout::xstep(c[0], dn) : dn * st;
+    const long s2 = f.axis == 0 ? Layout::xstep(c[0], 2 * dn) : 2 * dn * st;
+    Sb[o + s1] += 2.0 * total;
+    Sb[o + s2] += -total;
 }
 
 // Routing after a stage's transposed boundary application (Xbar in B): interior nodes feed the combine
@@ -557,24 +602,57 @@ void launchAdjRhs(hipStream_t s, const AdjArgs& a, double* rePartial) {
     hipLaunchKernelGGL(k_adj_rhs_b, grd, blk, 0, s, a, rePartial);
 }
 
-void launchAdjGhost(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, double* Sb) {
-    if (n > 0) hipLaunchKernelGGL(k_adj_ghost, dim3(nb(n, 64)), dim3(64), 0, s, L, P, G, n, Sb);
+void launchAdjGhost(hipStream_t s, const Layout& L, const Params& P, const GhostTables& G, int64_t n, double* Sb,
+                    double* contrib, const GatherPlan& donors) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_adj_ghost, dim3(nb(n, 64)), dim3(64), 0, s, L, P, G, n, Sb, contrib);
+    if (donors.n > 0) hipLaunchKernelGGL(k_adj_gather5, dim3(nb(donors.n, 128)), dim3(128), 0, s, L, donors, contrib, Sb);
 }
 
 void launchAdjFilter(hipStream_t s, const Layout& L, const Window& W, const int64_t* fn, int64_t n, double* Sb,
-                     double* tmp, const double* Pb, const double* PEb) {
+                     double* tmp, const double* Pb, const double* PEb, const GatherPlan& targets) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_adj_filter_take, dim3(nb(n, 128)), dim3(128), 0, s, L, fn, n, Sb, tmp);
-    hipLaunchKernelGGL(k_adj_filter_give, dim3(nb(n, 128)), dim3(128), 0, s, L, W, fn, n, Sb, tmp, Pb, PEb);
+    hipLaunchKernelGGL(k_adj_filter_take, dim3(nb(n, 128)), dim3(128), 0, s, L, W, fn, n, Sb, tmp, Pb, PEb);
+    hipLaunchKernelGGL(k_adj_filter_gather, dim3(nb(targets.n, 128)), dim3(128), 0, s, L, targets, tmp, Sb);
 }
 
 void launchAdjEdge(hipStream_t s, const Layout& L, const Params& P, const EdgeFace& f, double* Sb, const double* S,
-                   double ts, double* qb) {
+                   double ts, double* qb, double* wbuf) {
     const int ax1 = f.axis == 0 ? 1 : 0, ax2 = f.axis == 2 ? 1 : 2;
     const int lo[3] = {f.i0, f.j0, f.k0}, hi[3] = {f.i1, f.j1, f.k1};
     const int n1 = hi[ax1] - lo[ax1] + 1, n2 = hi[ax2] - lo[ax2] + 1;
     if (n1 <= 0 || n2 <= 0) return;
-    hipLaunchKernelGGL(k_adj_edge, dim3(nb(n1, 64), n2), dim3(64), 0, s, L, P, f, Sb, S, ts, qb);
+    if (f.type == NS_BC_INLET && !wbuf) throw std::invalid_argument("inlet adjoint needs its face buffer");
+    hipLaunchKernelGGL(k_adj_edge, dim3(nb(n1, 64), n2), dim3(64), 0, s, L, P, f, Sb, S, ts, qb, wbuf);
+    if (f.type == NS_BC_INLET)
+        hipLaunchKernelGGL(k_adj_inlet_gather, dim3(nb(n1, 64), n2), dim3(64), 0, s, L, f, Sb, (const double*)wbuf);
+}
+
+GatherPlan buildGatherPlan(std::vector<std::pair<int64_t, int32_t>> pairs, const std::function<void*(size_t)>& alloc) {
+    std::sort(pairs.begin(), pairs.end());
+    std::vector<int64_t> tgt, start;
+    std::vector<int32_t> src(pairs.size());
+    for (size_t e = 0; e < pairs.size(); ++e) {
+        if (e == 0 || pairs[e].first != pairs[e - 1].first) {
+            tgt.push_back(pairs[e].first);
+            start.push_back((int64_t)e);
+        }
+        src[e] = pairs[e].second;
+    }
+    start.push_back((int64_t)pairs.size());
+    GatherPlan p;
+    p.n = (int64_t)tgt.size();
+    p.tgt = (int64_t*)alloc(std::max<size_t>(1, tgt.size()) * sizeof(int64_t));
+    p.start = (int64_t*)alloc(start.size() * sizeof(int64_t));
+    p.src = (int32_t*)alloc(std::max<size_t>(1, src.size()) * sizeof(int32_t));
+    if (!tgt.empty()) {
+        if (hipMemcpy(p.tgt, tgt.data(), tgt.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p.src, src.data(), src.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("gather plan upload failed");
+    }
+    if (hipMemcpy(p.start, start.data(), start.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess)
+        throw std::runtime_error("gather plan upload failed");
+    return p;
 }
 
 void launchAdjRoute(hipStream_t s, const Layout& L, const uint8_t* cls, const double* B, double* Ub, double* const lam[4],

@@ -170,7 +170,38 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
         double* PEb = dalloc(wb);
         double* Ps[4], *PEs[4];
         for (int q = 0; q < 4; ++q) { Ps[q] = dalloc(wb); PEs[q] = dalloc(wb); }
-        double* ftmpA = dalloc(2 * std::max<int64_t>(1, nFilter) * sizeof(double));
+        double* ftmpA = dalloc(4 * std::max<int64_t>(1, nFilter) * sizeof(double));
+        // fixed-order sums of the ghost and immersed-filter transposes: the donors of every ghost's
+        // interpolation box, and the filter nodes with their six neighbours
+        auto dallocV = [&](size_t bytes) { return (void*)dalloc(bytes); };
+        double* gcontrib = dalloc((size_t)std::max<int64_t>(1, nGhost) * 40 * sizeof(double));
+        GatherPlan donors, filterTargets;
+        if (nGhost > 0) {
+            std::vector<int64_t> corner((size_t)nGhost * 8);
+            std::vector<uint8_t> gmask((size_t)nGhost);
+            hipCheck(hipMemcpy(corner.data(), G.corner, corner.size() * sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+            hipCheck(hipMemcpy(gmask.data(), G.mask, gmask.size(), hipMemcpyDeviceToHost), "copy");
+            std::vector<std::pair<int64_t, int32_t>> pairs;
+            for (int64_t g = 0; g < nGhost; ++g)
+                for (int c = 0; c < 8; ++c)
+                    if (!((gmask[g] >> c) & 1)) pairs.emplace_back(corner[8 * g + c], (int32_t)(8 * g + c));
+            donors = buildGatherPlan(std::move(pairs), dallocV);
+        }
+        if (nFilter > 0) {
+            std::vector<int64_t> fl((size_t)nFilter);
+            hipCheck(hipMemcpy(fl.data(), filterList, fl.size() * sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+            std::vector<std::pair<int64_t, int32_t>> pairs;
+            const long sy = L.pitch, sz = (long)L.plane;
+            for (int64_t q = 0; q < nFilter; ++q) {
+                const int64_t m = fl[q];
+                const int i = (int)(m % L.NI), j = (int)((m / L.NI) % L.NJ), k = (int)(m / ((int64_t)L.NI * L.NJ));
+                const int64_t o = (int64_t)L.off(i, j, k);
+                pairs.emplace_back(o, (int32_t)(2 * q + 1));
+                const long nbs[6] = {Layout::xstep(i, 1), Layout::xstep(i, -1), sy, -sy, sz, -sz};
+                for (long d : nbs) pairs.emplace_back(o + d, (int32_t)(2 * q));
+            }
+            filterTargets = buildGatherPlan(std::move(pairs), dallocV);
+        }
         const int nre = adjRhsPartials(L);
         double* rePart = dalloc((size_t)nre * sizeof(double));
         double reSum = 0;
@@ -180,6 +211,12 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
             if (f.type == NS_BC_INLET)
                 planeN = f.axis == 0 ? (int64_t)L.NJ * L.NK : (f.axis == 1 ? (int64_t)L.NI * L.NK : (int64_t)L.NI * L.NJ);
         double* qb = planeN ? dalloc((size_t)planeN * sizeof(double)) : nullptr;
+        int64_t inletN = 0;  // nodes of the largest inlet face (its in-plane filter's adjoint, per node)
+        for (const EdgeFace& f : faces)
+            if (f.type == NS_BC_INLET)
+                inletN = std::max<int64_t>(inletN, (int64_t)std::max(0, f.i1 - f.i0 + 1) * std::max(0, f.j1 - f.j0 + 1) *
+                                                       std::max(0, f.k1 - f.k0 + 1));
+        double* wbuf = inletN ? dalloc((size_t)inletN * sizeof(double)) : nullptr;
         if (qb) zero(qb, (size_t)planeN * sizeof(double));
         const AdjConst ac = makeAdjConst(P);
         const double vol = P.hx * P.hy * P.hz;
@@ -210,9 +247,9 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
         // transposed boundary application (edges in application order, filter, ghosts) at time ts, for
         // the state S it produced; Pbar/PEbar: adjoint of the pre-filter tables it recorded
         auto bcT = [&](double* Bv, const double* S, const double* Pbar, const double* PEbar, double ts) {
-            launchAdjGhost(stream, L, P, G, nGhost, Bv);
-            launchAdjFilter(stream, L, W, filterList, nFilter, Bv, ftmpA, Pbar, PEbar);
-            for (int q = 5; q >= 0; --q) launchAdjEdge(stream, L, P, faces[q], Bv, S, ts, qb);
+            launchAdjGhost(stream, L, P, G, nGhost, Bv, gcontrib, donors);
+            launchAdjFilter(stream, L, W, filterList, nFilter, Bv, ftmpA, Pbar, PEbar, filterTargets);
+            for (int q = 5; q >= 0; --q) launchAdjEdge(stream, L, P, faces[q], Bv, S, ts, qb, wbuf);
         };
         auto objAt = [&](int lv) { return lv >= n0 && lv <= n1 ? cJ : 0.0; };
 
