@@ -16,6 +16,7 @@
  * so it also produces an adjoint of U^{m-2} that reverse step m-1 adds to its result.
  */
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -176,6 +177,8 @@ void Case::adjointDrag(int nsteps, int n0, int n1, int ckpt, double* J, double* 
         auto dallocV = [&](size_t bytes) { return (void*)dalloc(bytes); };
         double* gcontrib = dalloc((size_t)std::max<int64_t>(1, nGhost) * 40 * sizeof(double));
         GatherPlan donors, filterTargets;
+        if (8 * nGhost > INT32_MAX || 2 * nFilter > INT32_MAX)
+            throw std::length_error("too many ghost or filter nodes for the adjoint's 32-bit gather lists");
         if (nGhost > 0) {
             std::vector<int64_t> corner((size_t)nGhost * 8);
             std::vector<uint8_t> gmask((size_t)nGhost);
