@@ -93,6 +93,27 @@ def test_persistent_stage_launches_equal_grid_launches(graph):
         assert np.array_equal(U0, U1), c
 
 
+def test_persistent_launch_with_fewer_tiles_than_ranges():
+    """One 24^3 case at 14-row x 128-plane tiles is a single tile (its two y-bands share one half-width
+    tile): seven of the eight XCD ranges are empty and all but one resident block find no tile; the state
+    still equals the grid launch's and the counters come back to zero (later steps would skip tiles
+    otherwise)."""
+    cfg = cases.sphere48(N=24, steps=6, precision="fp32")
+    out = []
+    for persistent in (0, 1):
+        b = nsfsi.Batch([cfg])
+        b.initialize()
+        b.setOption("graph", 1)
+        b.setOption("tile_ty", 14)
+        b.setOption("tile_kc", 128)
+        b.setOption("persistent", persistent)
+        b.step(6)
+        out.append((b.conserved(0), b.time(0)))
+        b.close()
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
+
+
 def test_batch_cases_advance_at_their_own_dt():
     cfgs = small_sweep(n=4, steps=5)
     b = nsfsi.Batch(cfgs)
